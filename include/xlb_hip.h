@@ -1,0 +1,222 @@
+/*
+ * xlb_hip.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann backend, libxlbhip.so.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns an int
+ * status: 0 = success, non-zero = failure with a message available from xlbh_last_error()
+ * (thread-local).  All field pointers are device pointers (HBM) unless stated otherwise.
+ * `stream` is a hipStream_t passed as void* (NULL = the legacy default stream); every
+ * launch is asynchronous on that stream, exactly as the reference's `wp.launch` is
+ * asynchronous on Warp's current stream.
+ *
+ * Field layout (identical to the reference's WarpGrid fields, xlb/grid/warp_grid.py:17-32):
+ * structure of arrays, C order, shape (card, nx, ny, nz) with z unit-stride; 2-D lattices use
+ * (card, nx, ny, 1).  `N = nx*ny*nz` is the number of cells.  Population l of cell
+ * (x,y,z) lives at element  l*N + (x*ny + y)*nz + z.
+ *
+ * Lattice direction order, weights and opposite indices follow
+ * xlb/velocity_set/d2q9.py:16-24, d3q19.py:17-30, d3q27.py:17-32.
+ */
+#ifndef XLB_HIP_H
+#define XLB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Lattices (xlb/velocity_set/d2q9.py, d3q19.py, d3q27.py). */
+enum { XLBH_D2Q9 = 0, XLBH_D3Q19 = 1, XLBH_D3Q27 = 2 };
+
+/* Collision operators (xlb/operator/collision/bgk.py, kbc.py). KBC: D2Q9 and D3Q27 only,
+ * as in kbc.py:183-185. */
+enum { XLBH_BGK = 0, XLBH_KBC = 1 };
+
+/* Storage formats of population fields.  The *_SHIFTED formats store the deviation
+ * f_l - w_l (no counterpart in the reference; its FP16 store is a plain cast,
+ * xlb/precision_policy.py:46-89). */
+enum {
+  XLBH_F32 = 0,
+  XLBH_F64 = 1,
+  XLBH_F16 = 2,
+  XLBH_BF16 = 3,
+  XLBH_F16_SHIFTED = 4,
+  XLBH_BF16_SHIFTED = 5
+};
+
+/* Arithmetic precision of the per-cell update (PrecisionPolicy.compute_precision). */
+enum { XLBH_COMPUTE_F32 = 0, XLBH_COMPUTE_F64 = 1 };
+
+/* Element types of macroscopic fields (rho, u, Pi) passed to the standalone operators. */
+enum { XLBH_FIELD_F32 = 0, XLBH_FIELD_F64 = 1, XLBH_FIELD_F16 = 2, XLBH_FIELD_BF16 = 3 };
+
+/* Boundary-condition kinds; one kind per boundary id (xlb/operator/boundary_condition/). */
+enum {
+  XLBH_BC_NONE = 0,
+  XLBH_BC_EQUILIBRIUM = 1,            /* bc_equilibrium.py:75-86            (streaming step) */
+  XLBH_BC_HALFWAY_BB = 2,             /* bc_halfway_bounce_back.py:67-85    (streaming step) */
+  XLBH_BC_FULLWAY_BB = 3,             /* bc_fullway_bounce_back.py:59-72    (collision step) */
+  XLBH_BC_DO_NOTHING = 4,             /* bc_do_nothing.py:52-62             (streaming step) */
+  XLBH_BC_ZOUHE_VELOCITY = 5,         /* bc_zouhe.py:279-312                (streaming step) */
+  XLBH_BC_ZOUHE_PRESSURE = 6,         /* bc_zouhe.py:314-338                (streaming step) */
+  XLBH_BC_REGULARIZED_VELOCITY = 7,   /* bc_regularized.py:134-168          (streaming step) */
+  XLBH_BC_REGULARIZED_PRESSURE = 8,   /* bc_regularized.py:170-196          (streaming step) */
+  XLBH_BC_EXTRAPOLATION_OUTFLOW = 9   /* bc_extrapolation_outflow.py:152-195 (streaming + post-collision) */
+};
+
+/* Body-force modes of the exact-difference forcing (xlb/operator/force/exact_difference_force.py). */
+enum { XLBH_FORCE_NONE = 0, XLBH_FORCE_UNIFORM = 1, XLBH_FORCE_FIELD = 2 };
+
+/* One entry of the boundary table: a device array of 256 entries indexed by boundary id
+ * (bc_mask value).  Id 0 = fluid, 255 = solid (cell skipped, nse_stepper.py:356-358).
+ * `rho`, `u` are the EquilibriumBC constants; other kinds ignore them. */
+typedef struct xlbh_bc_desc {
+  int32_t kind;
+  int32_t reserved;
+  double rho;
+  double u[3];
+} xlbh_bc_desc;
+
+/* ---------------------------------------------------------------------------------------
+ * Library
+ * ------------------------------------------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* xlbh_last_error(void);
+
+/* ABI version of this header (incremented on incompatible change). */
+int xlbh_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * The fused step.  Replaces IncompressibleNavierStokesStepper.warp_implementation and its
+ * kernel (xlb/operator/stepper/nse_stepper.py:245-392): pull streaming with periodic wrap,
+ * streaming-step BCs, macroscopic, quadratic equilibrium, BGK/KBC collision (+ exact-
+ * difference force), collision-step BCs, extrapolation-outflow auxiliary update, store.
+ *
+ *   f_in, f_out   : (Q, N) population fields in format `store`; f_out must not alias f_in.
+ *   bc_mask       : (N) uint8 boundary ids, or NULL when every cell is fluid.
+ *   missing_bits  : (N) uint32, bit l set <=> direction l is missing at the cell
+ *                   (packed form of the reference's (Q, N) bool missing_mask).  Read only
+ *                   at cells with a non-zero id.  May be NULL when bc_mask is NULL.
+ *   aux           : (N) prescribed values (velocity magnitude or density) for Zou-He and
+ *                   regularized cells, in the compute type (float or double); NULL if unused.
+ *                   Replaces the reference's stash in f_1[0] (boundary_condition.py:151-159,
+ *                   nse_stepper.py:318-342): the input field is never written.
+ *   bc_table      : device array of 256 xlbh_bc_desc, or NULL when bc_mask is NULL.
+ *   force_mode    : XLBH_FORCE_*; `force_uniform` (host, 3 doubles) for UNIFORM, `force_field`
+ *                   (device, (D, N) in the compute type) for FIELD.
+ *   omega         : relaxation rate.  timestep is accepted for interface parity (unused by
+ *                   the in-scope boundary conditions, as in the reference).
+ * ------------------------------------------------------------------------------------- */
+int xlbh_step(int lattice, int collision, int store, int compute,
+              const void* f_in, void* f_out,
+              const uint8_t* bc_mask, const uint32_t* missing_bits,
+              const void* aux, const xlbh_bc_desc* bc_table,
+              int force_mode, const double* force_uniform, const void* force_field,
+              int nx, int ny, int nz, double omega, int timestep, void* stream);
+
+/* n_steps fused steps with the buffer swap done in C (the caller's loop of
+ * mlups_3d.py:78-80 moved below the ABI).  Step t reads f_a when t is even and f_b when t is
+ * odd; after the call the newest populations are in f_b if n_steps is odd, else in f_a. */
+int xlbh_run(int lattice, int collision, int store, int compute,
+             void* f_a, void* f_b,
+             const uint8_t* bc_mask, const uint32_t* missing_bits,
+             const void* aux, const xlbh_bc_desc* bc_table,
+             int force_mode, const double* force_uniform, const void* force_field,
+             int nx, int ny, int nz, double omega, int timestep0, int n_steps, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Standalone operators (the reference's per-operator Warp kernels).
+ * ------------------------------------------------------------------------------------- */
+
+/* Pull streaming with periodic wrap: f_out[l](x) = f_in[l](x - c_l).  stream.py:59-99. */
+int xlbh_stream(int lattice, int store, const void* f_in, void* f_out,
+                int nx, int ny, int nz, void* stream);
+
+/* f = feq(rho, u).  rho: (1, N), u: (D, N) in field type `field`.  quadratic_equilibrium.py:35-98. */
+int xlbh_equilibrium(int lattice, int store, int compute, int field,
+                     const void* rho, const void* u, void* f,
+                     int nx, int ny, int nz, void* stream);
+
+/* Moments of f.  which: 0 = rho and u (macroscopic.py:28-66), 1 = rho only
+ * (zero_moment.py:22-47), 2 = u given rho (first_moment.py:25-66, rho is an input),
+ * 3 = second moment Pi (second_moment.py:68-101, written to `pi` with D(D+1)/2 rows). */
+int xlbh_moments(int lattice, int store, int compute, int field, int which,
+                 const void* f, void* rho, void* u, void* pi,
+                 int nx, int ny, int nz, void* stream);
+
+/* fout = collision(f, feq, rho, u, omega) (+ exact-difference force).  bgk.py:30-81,
+ * kbc.py:268-347, forced_collision.py:41-104.  rho, u: (1,N), (D,N) in type `field`. */
+int xlbh_collide(int lattice, int collision, int store, int compute, int field,
+                 const void* f, const void* feq, void* fout,
+                 const void* rho, const void* u, double omega,
+                 int force_mode, const double* force_uniform, const void* force_field,
+                 int nx, int ny, int nz, void* stream);
+
+/* Standalone boundary-condition application (BoundaryCondition._construct_kernel,
+ * boundary_condition.py:86-117): at cells with bc_mask == bc_id, f_post = functional(
+ * f_pre, f_post) of the streaming-step form of `kind`; elsewhere f_post is unchanged.
+ * `desc` is a HOST pointer to the BC constants. */
+int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id,
+                  const xlbh_bc_desc* desc,
+                  const void* f_pre, void* f_post,
+                  const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                  int nx, int ny, int nz, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Boundary masks (integer; bit-exact).
+ * ------------------------------------------------------------------------------------- */
+
+/* One boundary condition's indices -> bc_mask / missing mask, Warp semantics of
+ * IndicesBoundaryMasker (indices_boundary_masker.py:103-224): for each in-domain index p,
+ * bc_mask[p] = bc_id; for each l with p - c_l outside the domain, missing[l,p] = 1; if
+ * interior[i] (needs padding and p strictly inside the domain) then for each l with
+ * p - c_l inside, missing[l, p + c_l] = 1 and bc_mask[p + c_l] = bc_id.
+ *   indices : device int32 (3, n) (rows x, y, z; z row zeros for 2-D lattices)
+ *   interior: device uint8 (n)
+ *   missing_bool (Q, N) uint8 and/or missing_bits (N) uint32 are updated (either may be NULL).
+ * Calls for successive BCs on one stream apply in order, so a later BC's ids win. */
+int xlbh_mask_indices(int lattice, const int32_t* indices, const uint8_t* interior, int n,
+                      int bc_id, uint8_t* bc_mask, uint8_t* missing_bool, uint32_t* missing_bits,
+                      int nx, int ny, int nz, void* stream);
+
+/* (Q, N) bool missing mask -> (N) uint32 bits. */
+int xlbh_pack_missing(int lattice, const uint8_t* missing_bool, uint32_t* missing_bits,
+                      int nx, int ny, int nz, void* stream);
+
+/* Deterministic, ordered list of the cells whose bc_mask equals `bc_id` (bc_id < 0: every
+ * cell with 0 < id < 255).  `cells` must hold the worst case N entries; `count` is a device
+ * int32.  `workspace` must hold xlbh_compact_workspace_bytes(N) bytes. */
+int64_t xlbh_compact_workspace_bytes(int64_t n_cells);
+int xlbh_compact_cells(const uint8_t* bc_mask, int bc_id, int32_t* cells, int32_t* count,
+                       void* workspace, int nx, int ny, int nz, void* stream);
+
+/* aux[cells[i]] = values[i] for i < n (values in the compute type; `compute` selects it). */
+int xlbh_scatter_aux(int compute, const int32_t* cells, const void* values, int n, void* aux,
+                     void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Reductions (FP64 accumulation, deterministic: fixed-order two-pass).
+ * ------------------------------------------------------------------------------------- */
+
+/* Momentum-exchange force on boundary id `bc_id` whose streaming-step functional is `kind`
+ * (momentum_transfer.py:92-176).  f is the post-collision field (the stepper's newest
+ * output).  Result: out[0..D-1] (device doubles).  `workspace` must hold
+ * xlbh_reduce_workspace_bytes(N) bytes. */
+int64_t xlbh_reduce_workspace_bytes(int64_t n_cells);
+int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc_id,
+                           const xlbh_bc_desc* desc,
+                           const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                           const void* aux, double* out, void* workspace,
+                           int nx, int ny, int nz, void* stream);
+
+/* Global mass and momentum: out[0] = sum rho, out[1..3] = sum rho*u (= sum c f) over every
+ * cell whose bc_mask is not 255 (bc_mask may be NULL). */
+int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask,
+                       double* out, void* workspace, int nx, int ny, int nz, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XLB_HIP_H */

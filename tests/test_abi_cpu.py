@@ -1,0 +1,68 @@
+"""The C ABI library builds, loads without a GPU, and exports every entry point declared in
+include/xlb_hip.h with the signature the Python binding uses."""
+
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "xlb_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(xlbh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = header_functions()
+    for required in ("xlbh_step", "xlbh_run", "xlbh_stream", "xlbh_equilibrium", "xlbh_moments", "xlbh_collide",
+                     "xlbh_apply_bc", "xlbh_mask_indices", "xlbh_pack_missing", "xlbh_momentum_transfer",
+                     "xlbh_mass_momentum", "xlbh_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_header_symbol():
+    from xlb_amd import _lib
+
+    L = _lib.lib()
+    raw = ctypes.CDLL(_lib.LIB_PATH)
+    for name in header_functions():
+        assert hasattr(raw, name), f"{name} declared in xlb_hip.h but not exported"
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature in xlb_amd/_lib.py"
+        getattr(L, name)
+
+
+def test_error_channel_and_validation_without_gpu():
+    """Argument validation happens before any HIP call, so it runs on a machine without a GPU."""
+    from xlb_amd import _lib
+
+    L = _lib.lib()
+    assert L.xlbh_abi_version() == _lib.ABI_VERSION
+    # in-place step is rejected
+    buf = ctypes.c_void_p(0x1000)
+    rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, buf, buf, None, None, None, None,
+                     _lib.FORCE_NONE, None, None, 8, 8, 8, 1.0, 0, None)
+    assert rc != 0 and b"in place" in L.xlbh_last_error()
+    # 2-D lattice with nz > 1 is rejected
+    rc = L.xlbh_step(_lib.D2Q9, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000),
+                     None, None, None, None, _lib.FORCE_NONE, None, None, 8, 8, 4, 1.0, 0, None)
+    assert rc != 0 and b"2-D" in L.xlbh_last_error()
+    # KBC on D3Q19 is rejected (kbc.py:183-185)
+    rc = L.xlbh_collide(_lib.D3Q19, _lib.KBC, _lib.F32, _lib.COMPUTE_F32, _lib.FIELD_F32, ctypes.c_void_p(8),
+                        ctypes.c_void_p(16), ctypes.c_void_p(24), ctypes.c_void_p(32), ctypes.c_void_p(40), 1.0,
+                        _lib.FORCE_NONE, None, None, 4, 4, 4, None)
+    assert rc != 0 and b"KBC" in L.xlbh_last_error()
+    # unknown lattice
+    rc = L.xlbh_stream(7, _lib.F32, ctypes.c_void_p(8), ctypes.c_void_p(16), 4, 4, 4, None)
+    assert rc != 0 and b"lattice" in L.xlbh_last_error()
+    # workspace sizes
+    assert L.xlbh_reduce_workspace_bytes(256) == 4 * 8
+    assert L.xlbh_compact_workspace_bytes(512) == 2 * 2 * 4
+
+
+def test_bc_desc_layout_matches_header():
+    from xlb_amd import _lib
+
+    assert ctypes.sizeof(_lib.BcDesc) == 40
+    assert _lib.BcDesc.rho.offset == 8 and _lib.BcDesc.u.offset == 16

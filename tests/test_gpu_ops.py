@@ -1,0 +1,325 @@
+"""GPU parity of the standalone operators (HIP kernels via the C ABI) against the CPU oracle and
+the reference's known answers."""
+
+import numpy as np
+import pytest
+import torch
+
+import xlb_amd as xa
+from lbm_cases import decode, encode, from_np, oracle_masks, to_np
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+MI = xa.ComputeBackend.MI355X
+P = xa.PrecisionPolicy
+LAT = {"D2Q9": 0, "D3Q19": 1, "D3Q27": 2}
+
+
+def setup(lat_name, policy=P.FP32FP32):
+    vs = getattr(xa.velocity_set, lat_name)(policy, MI)
+    xa.init(velocity_set=vs, default_backend=MI, default_precision_policy=policy)
+    return vs
+
+
+def shape_for(vs, n):
+    return (n, n) if vs.d == 2 else (n, n, n)
+
+
+def sphere(n, d):
+    x = np.arange(n)
+    grids = np.meshgrid(*([x] * d), indexing="xy")
+    ind = np.where(sum((g - n // 2) ** 2 for g in grids) < (n // 4) ** 2)
+    return [tuple(int(v) for v in ind[i]) for i in range(d)]
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64, P.FP32FP16, P.FP32BF16_SHIFTED])
+def test_equilibrium_rest_state(lat, policy):
+    # tests/kernels/equilibrium/test_equilibrium_warp.py
+    vs = setup(lat, policy)
+    g = xa.grid.grid_factory(shape_for(vs, 20))
+    f = g.create_field(vs.q)
+    rho = g.create_field(1, fill_value=1.0, dtype=policy.compute_precision)
+    u = g.create_field(vs.d, fill_value=0.0, dtype=policy.compute_precision)
+    xa.operator.equilibrium.QuadraticEquilibrium()(rho, u, f)
+    fv = decode(to_np(f), policy.store_format, vs.w)
+    w = vs.w.reshape(-1, 1, 1, 1)
+    tol = {P.FP32FP32: 1e-7, P.FP64FP64: 1e-16, P.FP32FP16: 1e-3, P.FP32BF16_SHIFTED: 1e-7}[policy]
+    np.testing.assert_allclose(fv, np.broadcast_to(w, fv.shape), rtol=tol, atol=0)
+
+
+@pytest.mark.parametrize("lat,n,rho,vel", [("D2Q9", 100, 1.0, 0.0), ("D2Q9", 100, 1.1, 1.0), ("D2Q9", 50, 1.1, 2.0),
+                                            ("D3Q19", 50, 1.0, 0.0), ("D3Q19", 50, 1.1, 1.0), ("D3Q19", 50, 1.1, 2.0)])
+def test_macroscopic_inverts_equilibrium(lat, n, rho, vel):
+    # tests/kernels/macroscopic/test_macroscopic_warp.py:20-49
+    vs = setup(lat)
+    g = xa.grid.grid_factory(shape_for(vs, n))
+    rho_f = g.create_field(1, fill_value=rho)
+    u_f = g.create_field(vs.d, fill_value=vel)
+    feq = xa.operator.equilibrium.QuadraticEquilibrium()(rho_f, u_f, g.create_field(vs.q))
+    r, u = xa.operator.macroscopic.Macroscopic()(feq, g.create_field(1), g.create_field(vs.d))
+    assert np.allclose(r.cpu().numpy(), rho)
+    assert np.allclose(u.cpu().numpy(), vel, atol=1e-6)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+def test_moments_match_oracle(lat):
+    vs = setup(lat, P.FP64FP64)
+    g = xa.grid.grid_factory(shape_for(vs, 12))
+    rng = np.random.default_rng(0)
+    fnp = rng.random((vs.q,) + g.field_shape)
+    f = torch.from_numpy(fnp).cuda()
+    r, u = xa.operator.macroscopic.Macroscopic()(f, g.create_field(1), g.create_field(vs.d))
+    orho, ou = orc.macroscopic(vs.lattice_id, orc.F64, 1, fnp, vs.d)
+    np.testing.assert_allclose(r.cpu().numpy(), orho, rtol=1e-14)
+    np.testing.assert_allclose(u.cpu().numpy(), ou, rtol=1e-13, atol=1e-15)
+    pi = xa.operator.macroscopic.SecondMoment()(f, g.create_field(vs.d * (vs.d + 1) // 2))
+    expect = np.tensordot(vs.cc.T, fnp, axes=(1, 0))
+    np.testing.assert_allclose(pi.cpu().numpy(), expect, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("lat,n", [("D2Q9", 50), ("D3Q19", 30), ("D3Q27", 24)])
+def test_stream_is_roll(lat, n):
+    # tests/kernels/stream/test_stream_warp.py:21-77
+    vs = setup(lat)
+    g = xa.grid.grid_factory(shape_for(vs, n) if vs.d == 3 else (n, n + 3))
+    f = g.create_field(vs.q)
+    f[..., f.shape[-1] // 2] = 1
+    if vs.d == 2:
+        f[:, :, (f.shape[2] // 3), :] = 2
+    out = xa.operator.stream.Stream()(f, g.create_field(vs.q))
+    fnp = f.cpu().numpy()
+    for i in range(vs.q):
+        shift = tuple(int(v) for v in vs.c[:, i]) + ((0,) if vs.d == 2 else ())
+        np.testing.assert_array_equal(out[i].cpu().numpy(), np.roll(fnp[i], shift, axis=(0, 1, 2)))
+
+
+@pytest.mark.parametrize("lat,omega", [("D2Q9", 0.6), ("D2Q9", 1.0), ("D3Q19", 0.6), ("D3Q19", 1.0), ("D3Q27", 1.0)])
+def test_bgk_closed_form(lat, omega):
+    # tests/kernels/collision/test_bgk_collision_warp.py:20-53
+    vs = setup(lat)
+    g = xa.grid.grid_factory(shape_for(vs, 20))
+    rng = np.random.default_rng(1)
+    f = torch.from_numpy(rng.random((vs.q,) + g.field_shape).astype(np.float32)).cuda()
+    feq = torch.from_numpy(rng.random((vs.q,) + g.field_shape).astype(np.float32)).cuda()
+    out = xa.operator.collision.BGK()(f, feq, g.create_field(vs.q), None, None, omega)
+    expect = f.cpu().numpy() - np.float32(omega) * (f.cpu().numpy() - feq.cpu().numpy())
+    np.testing.assert_allclose(out.cpu().numpy(), expect, atol=1e-5)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q27"])
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+def test_kbc_matches_oracle(lat, policy):
+    vs = setup(lat, policy)
+    g = xa.grid.grid_factory(shape_for(vs, 10))
+    rng = np.random.default_rng(2)
+    shp = g.field_shape
+    rho = 1 + 0.05 * rng.standard_normal(shp)
+    u = 0.05 * rng.standard_normal((vs.d,) + shp)
+    fmt, comp = policy.store_format, policy.compute_id
+    feq = np.zeros((vs.q,) + shp, dtype=np.float32 if fmt == orc.F32 else np.float64)
+    orc.equilibrium(vs.lattice_id, fmt, comp, rho, u, feq)
+    f = (feq + 1e-3 * rng.standard_normal(feq.shape) * vs.w.reshape(-1, 1, 1, 1)).astype(feq.dtype)
+    expect = orc.collide(vs.lattice_id, 1, fmt, comp, f, feq, rho, u, 1.7)
+    dt = policy.compute_precision.torch_dtype
+    out = xa.operator.collision.KBC()(torch.from_numpy(f).cuda(), torch.from_numpy(feq).cuda(), g.create_field(vs.q),
+                                      torch.from_numpy(rho[None]).to(dt).cuda(), torch.from_numpy(u).to(dt).cuda(), 1.7)
+    tol = 1e-6 if policy == P.FP32FP32 else 1e-14
+    np.testing.assert_allclose(out.cpu().numpy(), expect, atol=tol)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+def test_forced_collision_matches_oracle(lat):
+    vs = setup(lat, P.FP64FP64)
+    g = xa.grid.grid_factory(shape_for(vs, 8))
+    rng = np.random.default_rng(3)
+    shp = g.field_shape
+    rho = 1 + 0.05 * rng.standard_normal(shp)
+    u = 0.05 * rng.standard_normal((vs.d,) + shp)
+    feq = np.zeros((vs.q,) + shp)
+    orc.equilibrium(vs.lattice_id, orc.F64, 1, rho, u, feq)
+    f = feq + 1e-3 * rng.standard_normal(feq.shape)
+    F = np.array([1e-3, -2e-3, 5e-4][: vs.d])
+    expect = orc.collide(vs.lattice_id, 0, orc.F64, 1, f, feq, rho, u, 1.3, force=list(F))
+    op = xa.operator.collision.ForcedCollision(xa.operator.collision.BGK(), force_vector=F)
+    out = op(torch.from_numpy(f).cuda(), torch.from_numpy(feq).cuda(), g.create_field(vs.q),
+             torch.from_numpy(rho[None]).cuda(), torch.from_numpy(u).cuda(), 1.3)
+    np.testing.assert_allclose(out.cpu().numpy(), expect, atol=1e-15)
+
+
+def _mask_case(vs, n, bcs_fn):
+    g = xa.grid.grid_factory(shape_for(vs, n))
+    bcs = bcs_fn(g)
+    bc_mask = g.create_field(1, dtype=xa.Precision.UINT8)
+    missing = g.create_field(vs.q, dtype=xa.Precision.BOOL)
+    xa.operator.boundary_masker.IndicesBoundaryMasker()(bcs, bc_mask, missing)
+    return g, bcs, bc_mask, missing
+
+
+@pytest.mark.parametrize("lat,n", [("D2Q9", 50), ("D2Q9", 100), ("D3Q19", 50), ("D3Q27", 50)])
+def test_masker_matches_reference_test(lat, n):
+    # tests/boundary_conditions/mask/test_bc_indices_masker_warp.py:26-92
+    vs = setup(lat)
+    ind = sphere(n, vs.d)
+    g, bcs, bc_mask, missing = _mask_case(vs, n, lambda g: [xa.operator.boundary_condition.FullwayBounceBackBC(indices=ind)])
+    assert bc_mask.dtype == torch.uint8 and missing.dtype == torch.bool
+    assert tuple(bc_mask.shape) == (1,) + g.field_shape and tuple(missing.shape) == (vs.q,) + g.field_shape
+    bm = bc_mask.cpu().numpy()
+    assert np.all(bm[(0,) + tuple(ind)] == bcs[0].id)
+    bm[(0,) + tuple(ind)] = 0
+    assert np.all(bm == 0)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+def test_masker_bit_exact_vs_oracle(lat):
+    """bc_mask, missing mask and packed bits bit-exact against the oracle, with padded interior
+    geometry, faces with and without edges, and two BCs touching the same cells (later wins)."""
+    vs = setup(lat)
+    n = 24
+
+    def bcs_fn(g):
+        BC = xa.operator.boundary_condition
+        box = g.bounding_box_indices()
+        ne = g.bounding_box_indices(remove_edges=True)
+        faces = ["bottom", "top"] if vs.d == 2 else ["bottom", "top", "front", "back"]
+        walls = np.unique(np.array([sum((box[k][i] for k in faces), []) for i in range(vs.d)]), axis=-1).tolist()
+        sph = sphere(n, vs.d)
+        near = [tuple(int(v) + 2 if i == 0 else int(v) for v in sph[i]) for i in range(vs.d)]
+        return [BC.FullwayBounceBackBC(indices=walls), BC.ZouHeBC("velocity", prescribed_value=(0.03,) + (0.0,) * (vs.d - 1), indices=ne["left"]),
+                BC.ExtrapolationOutflowBC(indices=ne["right"]), BC.HalfwayBounceBackBC(indices=sph),
+                BC.HalfwayBounceBackBC(indices=near)]
+
+    g, bcs, bc_mask, missing = _mask_case(vs, n, bcs_fn)
+    o_mask, o_missing = oracle_masks(vs.lattice_id, vs.q, vs.d, g.field_shape, bcs)
+    np.testing.assert_array_equal(bc_mask.cpu().numpy(), o_mask)
+    np.testing.assert_array_equal(missing.cpu().numpy().astype(np.uint8), o_missing)
+    from xlb_amd.operator.boundary_masker.packing import packed_missing
+
+    bits = packed_missing(missing, vs.lattice_id).cpu().numpy().view(np.uint32)
+    expect = np.zeros(bits.shape, dtype=np.uint32)
+    for l in range(vs.q):
+        expect |= o_missing[l].reshape(-1).astype(np.uint32) << np.uint32(l)
+    np.testing.assert_array_equal(bits, expect)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19"])
+def test_equilibrium_bc_known_answer(lat):
+    # tests/boundary_conditions/bc_equilibrium/test_bc_equilibrium_warp.py:19-93
+    vs = setup(lat)
+    n = 50 if vs.d == 3 else 100
+    ind = sphere(n, vs.d)
+    BC = xa.operator.boundary_condition
+    eq = BC.EquilibriumBC(rho=1.0, u=(0.0,) * vs.d, equilibrium_operator=xa.operator.equilibrium.QuadraticEquilibrium(), indices=ind)
+    g, bcs, bc_mask, missing = _mask_case(vs, n, lambda g: [eq])
+    f_pre = g.create_field(vs.q, dtype=xa.Precision.FP32)
+    f_post = g.create_field(vs.q, dtype=xa.Precision.FP32, fill_value=2.0)
+    f = eq(f_pre, f_post, bc_mask, missing).cpu().numpy()
+    inside = np.zeros(g.field_shape, bool)
+    inside[tuple(ind) + ((0,) if vs.d == 2 else ())] = True
+    for i, w in enumerate(vs.w):
+        assert np.allclose(f[i][inside], w)
+        assert np.all(f[i][~inside] == 2.0)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+def test_fullway_bc_known_answer(lat):
+    # tests/boundary_conditions/bc_fullway_bounce_back/test_bc_fullway_bounce_back_warp.py,
+    # with the flip asserted.
+    vs = setup(lat)
+    n = 50
+    ind = sphere(n, vs.d)
+    BC = xa.operator.boundary_condition
+    fw = BC.FullwayBounceBackBC(indices=ind)
+    g, bcs, bc_mask, missing = _mask_case(vs, n, lambda g: [fw])
+    rng = np.random.default_rng(4)
+    pre = rng.random((vs.q,) + g.field_shape).astype(np.float32)
+    f_pre = torch.from_numpy(pre).cuda()
+    f_post = g.create_field(vs.q, dtype=xa.Precision.FP32, fill_value=2.0)
+    f = fw(f_pre, f_post, bc_mask, missing).cpu().numpy()
+    inside = np.zeros(g.field_shape, bool)
+    inside[tuple(ind) + ((0,) if vs.d == 2 else ())] = True
+    for i in range(vs.q):
+        np.testing.assert_array_equal(f[i][inside], pre[vs.opp_indices[i]][inside])
+        assert np.all(f[i][~inside] == 2.0)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+@pytest.mark.parametrize("which", ["halfway", "do_nothing", "zouhe_v", "zouhe_p", "reg_v", "reg_p", "extrap"])
+def test_standalone_bc_matches_oracle(lat, which):
+    vs = setup(lat, P.FP64FP64)
+    n = 14
+    BC = xa.operator.boundary_condition
+
+    def make(g):
+        ne = g.bounding_box_indices(remove_edges=True)
+        side = ne["left"]
+        v = (0.04,) + (0.0,) * (vs.d - 1)
+        return {
+            "halfway": lambda: BC.HalfwayBounceBackBC(indices=sphere(n, vs.d)),
+            "do_nothing": lambda: BC.DoNothingBC(indices=side),
+            "zouhe_v": lambda: BC.ZouHeBC("velocity", prescribed_value=v, indices=side),
+            "zouhe_p": lambda: BC.ZouHeBC("pressure", prescribed_value=1.01, indices=side),
+            "reg_v": lambda: BC.RegularizedBC("velocity", profile=lambda idx: 0.01 + 0.001 * idx[1], indices=side),
+            "reg_p": lambda: BC.RegularizedBC("pressure", prescribed_value=0.99, indices=side),
+            "extrap": lambda: BC.ExtrapolationOutflowBC(indices=ne["right"]),
+        }[which]()
+
+    g, bcs, bc_mask, missing = _mask_case(vs, n, lambda g: [make(g)])
+    bc = bcs[0]
+    rng = np.random.default_rng(5)
+    w = vs.w.reshape(-1, 1, 1, 1)
+    pre = w * (1 + 0.05 * rng.standard_normal((vs.q,) + g.field_shape))
+    post = w * (1 + 0.05 * rng.standard_normal((vs.q,) + g.field_shape))
+    out = bc(torch.from_numpy(pre).cuda(), torch.from_numpy(post).cuda(), bc_mask, missing).cpu().numpy()
+    aux = None
+    if bc.needs_aux_init:
+        aux = bc.aux.cpu().numpy()
+    desc = dict(id=bc.id, kind=bc.kind)
+    expect = orc.apply_bc(vs.lattice_id, orc.F64, 1, desc, pre, post, bc_mask.cpu().numpy(),
+                          missing.cpu().numpy().astype(np.uint8), aux)
+    np.testing.assert_allclose(out, expect, rtol=1e-13, atol=1e-16)
+
+
+@pytest.mark.parametrize("lat", ["D3Q19", "D3Q27"])
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+def test_momentum_transfer_matches_oracle(lat, policy):
+    from lbm_cases import Pair
+
+    pair = Pair("sphere3d" if lat == "D3Q27" else "sphere3d_bgk", policy)
+    pair.step_product(30)
+    pair.step_oracle(30)
+    sph = pair.bcs[-1]
+    mt = xa.operator.force.MomentumTransfer(sph)
+    F = mt(pair.f0, pair.f1, pair.bc_mask, pair.missing)
+    expect = orc.momentum_transfer(pair.lat, pair.fmt, pair.comp, dict(id=sph.id, kind=sph.kind), pair.o_f0, pair.o_mask,
+                                   pair.o_missing)
+    assert F.shape == (3,) and F.dtype == np.float64
+    # FP32: per-cell sums round differently on the two sides (FMA on the GPU): bound relative to |F|
+    atol = 1e-5 * np.abs(expect).max() if policy == P.FP32FP32 else 1e-12
+    np.testing.assert_allclose(F, expect[:3], rtol=0, atol=atol)
+    assert F[0] > 0  # drag along +x
+    # deterministic: identical on a second call
+    np.testing.assert_array_equal(mt(pair.f0, pair.f1, pair.bc_mask, pair.missing), F)
+
+
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP32BF16_SHIFTED, P.FP64FP64])
+def test_mass_momentum_reduction(policy):
+    vs = setup("D3Q19", policy)
+    g = xa.grid.grid_factory((40, 33, 17))
+    rng = np.random.default_rng(6)
+    shp = g.field_shape
+    f = np.zeros((vs.q,) + shp, dtype={0: np.float32, 1: np.float64, 5: np.uint16}[policy.store_format])
+    orc.equilibrium(1, policy.store_format, policy.compute_id, 1 + 0.01 * rng.standard_normal(shp),
+                    0.02 * rng.standard_normal((3,) + shp), f)
+    fv = decode(f, policy.store_format, vs.w)
+    ft = from_np(f, g.create_field(vs.q))
+    from xlb_amd import _lib
+
+    out = torch.zeros(4, dtype=torch.float64, device="cuda")
+    ws = torch.empty(int(_lib.lib().xlbh_reduce_workspace_bytes(ft[0].numel())), dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().xlbh_mass_momentum(1, policy.store_format, _lib.ptr(ft), None, _lib.ptr(out), _lib.ptr(ws),
+                                             *shp, _lib.stream_of(ft)))
+    got = out.cpu().numpy()
+    np.testing.assert_allclose(got[0], fv.sum(), rtol=1e-13)
+    np.testing.assert_allclose(got[1:], np.tensordot(vs.c, fv, axes=(1, 0)).sum(axis=(1, 2, 3)), rtol=1e-10, atol=1e-10)

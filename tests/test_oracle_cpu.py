@@ -1,0 +1,391 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+Checks come from (a) the reference's own known-answer tests (tests/kernels/*, tests/boundary_conditions/*),
+(b) lattice identities, (c) conservation laws and closed-form flows (plane Poiseuille, Taylor-Green
+decay), and (d) special cases that reduce to a library routine (numpy.roll, numpy/torch rounding).
+Each is chosen so that a plausible slip -- a dropped term, a wrong sign or index, a transposed
+operand -- fails at least one of them.
+"""
+
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+
+LATS = {0: (2, 9), 1: (3, 19), 2: (3, 27)}
+
+
+def ref_lattice(lat):
+    """The reference's constructors (d2q9.py:16-24, d3q19.py:17-30, d3q27.py:17-32)."""
+    if lat == 0:
+        cx = [0, 0, 0, 1, -1, 1, -1, 1, -1]
+        cy = [0, 1, -1, 0, 1, -1, 0, 1, -1]
+        c = np.array(tuple(zip(cx, cy))).T
+        w = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 9, 1 / 36, 1 / 36])
+        return c, w
+    full = list(itertools.product([0, -1, 1], repeat=3))
+    if lat == 1:
+        c = np.array([ci for ci in full if np.sum(np.abs(ci)) <= 2]).T
+        w = np.array([{0: 1 / 3, 1: 1 / 18, 2: 1 / 36}[int(np.sum(np.abs(c[:, i])))] for i in range(19)])
+    else:
+        c = np.array(full).T
+        w = np.array([{0: 8 / 27, 1: 2 / 27, 2: 1 / 54, 3: 1 / 216}[int(np.sum(np.abs(c[:, i])))] for i in range(27)])
+    return c, w
+
+
+def grid_shape(lat, n):
+    return (n, n, 1) if lat == 0 else (n, n, n)
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+def test_lattice_tables(lat):
+    d, q = LATS[lat]
+    t = orc.lattice(lat)
+    c, w = ref_lattice(lat)
+    assert t["c"][:d].tolist() == c.tolist()
+    np.testing.assert_allclose(t["w"], w, rtol=0, atol=1e-16)
+    opp = t["opp"]
+    assert (c[:, opp] == -c).all() and (opp[opp] == np.arange(q)).all()
+    # isotropy: sum w = 1, sum w c_a c_b = delta/3, odd moments vanish
+    assert abs(w.sum() - 1) < 1e-15
+    np.testing.assert_allclose((w * c) @ c.T, np.eye(d) / 3, atol=1e-15)
+    np.testing.assert_allclose(w @ c.T, 0, atol=1e-16)
+    # qi = cc - delta/3 with off-diagonal doubled
+    k = 0
+    for a in range(d):
+        for b in range(a, d):
+            expect = c[a] * c[b] - (1 / 3 if a == b else 0)
+            if a != b:
+                expect = 2 * expect
+            np.testing.assert_allclose(t["qi"][:, k], expect, atol=1e-15)
+            k += 1
+
+
+def test_f16_bf16_rounding_matches_numpy_and_torch():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.standard_normal(20000).astype(np.float32) * 10.0 ** rng.integers(-9, 5, 20000),
+        np.array([0.0, -0.0, 65504, 65519.99, 65520, 1e-8, 2.0**-24, 2.0**-25, 1.5 * 2.0**-24, np.inf, -np.inf], np.float32),
+    ]).astype(np.float32)
+    with np.errstate(over="ignore"):
+        ref16 = x.astype(np.float16).view(np.uint16)
+    assert (orc.f32_to_f16_bits(x) == ref16).all()
+    tb = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert (orc.f32_to_bf16_bits(x) == tb).all()
+
+
+# ------------------------------------------------------------------ reference known answers
+@pytest.mark.parametrize("lat,n", [(0, 50), (1, 20), (2, 12)])
+def test_equilibrium_rest_state_is_weights(lat, n):
+    # tests/kernels/equilibrium/test_equilibrium_warp.py: feq(1, 0) = w, sum = 1
+    d, q = LATS[lat]
+    shp = grid_shape(lat, n)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, np.ones(shp), np.zeros((d,) + shp), f)
+    _, w = ref_lattice(lat)
+    np.testing.assert_allclose(f, np.broadcast_to(w.reshape(-1, 1, 1, 1), f.shape), atol=1e-16)
+    np.testing.assert_allclose(f.sum(0), 1.0, atol=1e-15)
+
+
+@pytest.mark.parametrize("lat,rho,vel", [(0, 1.0, 0.0), (0, 1.1, 1.0), (0, 1.1, 2.0), (1, 1.0, 0.0), (1, 1.1, 1.0), (1, 1.1, 2.0), (2, 1.1, 1.0)])
+def test_macroscopic_inverts_equilibrium(lat, rho, vel):
+    # tests/kernels/macroscopic/test_macroscopic_warp.py (atol 1e-6 on u), FP32 store/compute
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 10)
+    f = np.zeros((q,) + shp, dtype=np.float32)
+    orc.equilibrium(lat, orc.F32, 0, np.full(shp, rho), np.full((d,) + shp, vel), f)
+    r, u = orc.macroscopic(lat, orc.F32, 0, f, d)
+    np.testing.assert_allclose(r, rho, rtol=1e-6)
+    np.testing.assert_allclose(u, vel, atol=1e-6)
+
+
+@pytest.mark.parametrize("lat,omega", [(0, 0.6), (0, 1.0), (1, 0.6), (1, 1.0), (2, 1.0)])
+def test_bgk_closed_form(lat, omega):
+    # tests/kernels/collision/test_bgk_collision_warp.py: fout = f - omega (f - feq)
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 8)
+    rng = np.random.default_rng(1)
+    f = rng.random((q,) + shp).astype(np.float32)
+    feq = rng.random((q,) + shp).astype(np.float32)
+    out = orc.collide(lat, 0, orc.F32, 0, f, feq, None, None, omega)
+    np.testing.assert_allclose(out, f - np.float32(omega) * (f - feq), atol=1e-6)
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+def test_stream_is_roll(lat):
+    # tests/kernels/stream/test_stream_warp.py: stream = np.roll(f_i, c_i)
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 9) if lat else (11, 7, 1)
+    rng = np.random.default_rng(2)
+    f = rng.random((q,) + shp)
+    out = orc.stream(lat, orc.F64, f)
+    c, _ = ref_lattice(lat)
+    for i in range(q):
+        shift = tuple(int(v) for v in c[:, i]) + ((0,) if d == 2 else ())
+        np.testing.assert_array_equal(out[i], np.roll(f[i], shift, axis=(0, 1, 2)))
+
+
+def _sphere(n, d):
+    x = np.arange(n)
+    grids = np.meshgrid(*([x] * d), indexing="xy")
+    r2 = sum((g - n // 2) ** 2 for g in grids)
+    return np.where(r2 < (n // 4) ** 2)
+
+
+@pytest.mark.parametrize("lat,n", [(0, 50), (1, 20), (2, 16)])
+def test_masker_no_padding_marks_exactly_the_indices(lat, n):
+    # tests/boundary_conditions/mask/test_bc_indices_masker_warp.py (FullwayBB: no padding)
+    d, q = LATS[lat]
+    shp = grid_shape(lat, n)
+    ind = _sphere(n, d)
+    idx = np.array(ind + ((np.zeros(len(ind[0]), int),) if d == 2 else ()))
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.zeros(idx.shape[1], np.uint8), 5, bc_mask, miss)
+    expect = np.zeros(shp, np.uint8)
+    expect[tuple(idx)] = 5
+    np.testing.assert_array_equal(bc_mask[0], expect)
+    assert miss.sum() == 0  # sphere strictly interior: no pull leaves the domain
+
+
+@pytest.mark.parametrize("lat", [1, 2])
+def test_masker_padding_marks_solid_neighbours(lat):
+    """Halfway-BB semantics (indices_boundary_masker.py:148-156): missing[l, x] iff x - c_l is solid."""
+    d, q = LATS[lat]
+    n = 12
+    shp = (n, n, n)
+    ind = _sphere(n, 3)
+    idx = np.array(ind)
+    solid = np.zeros(shp, bool)
+    solid[tuple(idx)] = True
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.ones(idx.shape[1], np.uint8), 3, bc_mask, miss)
+    c, _ = ref_lattice(lat)
+    for l in range(q):
+        expect = np.roll(solid, tuple(int(v) for v in c[:, l]), axis=(0, 1, 2))  # solid at x - c_l
+        np.testing.assert_array_equal(miss[l].astype(bool), expect)
+    dil = np.zeros(shp, bool)
+    for l in range(q):
+        dil |= np.roll(solid, tuple(int(v) for v in c[:, l]), axis=(0, 1, 2))
+    np.testing.assert_array_equal(bc_mask[0] == 3, dil)
+
+
+@pytest.mark.parametrize("lat", [0, 1])
+def test_equilibrium_bc_sets_weights_inside_only(lat):
+    # tests/boundary_conditions/bc_equilibrium/test_bc_equilibrium_warp.py
+    d, q = LATS[lat]
+    n = 20
+    shp = grid_shape(lat, n)
+    ind = _sphere(n, d)
+    idx = np.array(ind + ((np.zeros(len(ind[0]), int),) if d == 2 else ()))
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.zeros(idx.shape[1], np.uint8), 1, bc_mask, miss)
+    fpre = np.zeros((q,) + shp, np.float32)
+    fpost = np.full((q,) + shp, 2.0, np.float32)
+    out = orc.apply_bc(lat, orc.F32, 0, dict(id=1, kind=orc.EQUILIBRIUM, rho=1.0, u=(0, 0, 0)), fpre, fpost, bc_mask, miss)
+    _, w = ref_lattice(lat)
+    inside = bc_mask[0] == 1
+    for i in range(q):
+        np.testing.assert_allclose(out[i][inside], w[i], rtol=1e-7)
+        np.testing.assert_array_equal(out[i][~inside], 2.0)
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+def test_fullway_bounce_back_flips_inside_only(lat):
+    # tests/boundary_conditions/bc_fullway_bounce_back/*: the flip the reference computes but never asserts
+    d, q = LATS[lat]
+    n = 16
+    shp = grid_shape(lat, n)
+    ind = _sphere(n, d)
+    idx = np.array(ind + ((np.zeros(len(ind[0]), int),) if d == 2 else ()))
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.zeros(idx.shape[1], np.uint8), 2, bc_mask, miss)
+    rng = np.random.default_rng(3)
+    fpre = rng.random((q,) + shp).astype(np.float32)
+    fpost = np.full((q,) + shp, 2.0, np.float32)
+    out = orc.apply_bc(lat, orc.F32, 0, dict(id=2, kind=orc.FULLWAY_BB), fpre, fpost, bc_mask, miss)
+    opp = orc.lattice(lat)["opp"]
+    inside = bc_mask[0] == 2
+    for i in range(q):
+        np.testing.assert_array_equal(out[i][inside], fpre[opp[i]][inside])
+        np.testing.assert_array_equal(out[i][~inside], 2.0)
+
+
+# ------------------------------------------------------------------ conservation and closed forms
+def _run(lat, coll, f0, omega, steps, bc_mask=None, miss=None, bcs=(), force=None, store=orc.F64, compute=1):
+    f0 = f0.copy()
+    f1 = f0.copy()
+    for _ in range(steps):
+        orc.step(lat, coll, store, compute, f0, f1, bc_mask, miss, list(bcs), omega, force=force)
+        f0, f1 = f1, f0
+    return f0
+
+
+@pytest.mark.parametrize("lat,coll", [(1, 0), (2, 1), (0, 1), (0, 0)])
+def test_periodic_mass_and_momentum_conserved(lat, coll):
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 10)
+    rng = np.random.default_rng(4)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, 1 + 0.05 * rng.standard_normal(shp), 0.05 * rng.standard_normal((d,) + shp), f)
+    f += 1e-3 * rng.standard_normal(f.shape) * ref_lattice(lat)[1].reshape(-1, 1, 1, 1)
+    c, _ = ref_lattice(lat)
+    m0, p0 = f.sum(), np.tensordot(c, f, axes=(1, 0)).sum(axis=(1, 2, 3))
+    out = _run(lat, coll, f, 1.7, 20)
+    assert abs(out.sum() - m0) < 1e-11 * m0
+    np.testing.assert_allclose(np.tensordot(c, out, axes=(1, 0)).sum(axis=(1, 2, 3)), p0, atol=1e-11)
+
+
+@pytest.mark.parametrize("lat,coll", [(1, 0), (2, 1), (0, 1)])
+def test_uniform_flow_is_steady(lat, coll):
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 6)
+    u0 = np.array([0.03, -0.02, 0.01][:d]).reshape((d, 1, 1, 1)) * np.ones((d,) + shp)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, np.ones(shp), u0, f)
+    out = _run(lat, coll, f, 1.3, 10)
+    np.testing.assert_allclose(out, f, atol=1e-15)
+
+
+def test_closed_box_halfway_conserves_mass_and_zero_wall_flux():
+    lat, (d, q) = 1, LATS[1]
+    n = 10
+    shp = (n, n, n)
+    g = np.indices(shp)
+    faces = (g[0] == 0) | (g[0] == n - 1) | (g[1] == 0) | (g[1] == n - 1) | (g[2] == 0) | (g[2] == n - 1)
+    idx = np.array(np.where(faces))
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.zeros(idx.shape[1], np.uint8), 1, bc_mask, miss)
+    rng = np.random.default_rng(5)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, 1 + 0.02 * rng.standard_normal(shp), 0.03 * rng.standard_normal((d,) + shp), f)
+    m0 = f.sum()
+    out = _run(lat, 0, f, 1.4, 30, bc_mask, miss, [dict(id=1, kind=orc.HALFWAY_BB)])
+    assert abs(out.sum() - m0) < 1e-11 * m0
+
+
+def test_plane_poiseuille_d2q9():
+    """Body-force-driven channel with halfway bounce-back walls at y = -1/2 and ny - 1/2; at
+    tau = 1/2 + sqrt(3/16) the BGK halfway wall is exactly mid-link, so the steady profile is
+    the parabola u(y) = g/(2 nu) (y + 1/2)(ny - 1/2 - y)."""
+    lat, (d, q) = 0, LATS[0]
+    nx, ny = 4, 16
+    shp = (nx, ny, 1)
+    tau = 0.5 + np.sqrt(3 / 16)
+    omega, nu = 1 / tau, (tau - 0.5) / 3
+    # The masker marks every pull that leaves the box (also across the periodic x ends), so the
+    # periodic channel's mask is built directly: missing iff the pull crosses a y wall.
+    c, _ = ref_lattice(lat)
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    bc_mask[0, :, 0, :] = 1
+    bc_mask[0, :, ny - 1, :] = 1
+    miss = np.zeros((q,) + shp, np.uint8)
+    ys = np.arange(ny).reshape(1, ny, 1)
+    for l in range(q):
+        src = ys - c[1, l]
+        miss[l] = np.broadcast_to((src < 0) | (src >= ny), shp)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, np.ones(shp), np.zeros((d,) + shp), f)
+    F = 1e-6
+    out = _run(lat, 0, f, omega, 6000, bc_mask, miss, [dict(id=1, kind=orc.HALFWAY_BB)], force=(F, 0.0))
+    rho = out.sum(0)
+    ux = (np.tensordot(c, out, axes=(1, 0))[0] / rho)[0, :, 0]
+    y = np.arange(ny)
+    expect = F / (2 * nu) * (y + 0.5) * (ny - 0.5 - y)
+    # With exact-difference forcing the pre-collision velocity sits F/2 above the parabola.
+    np.testing.assert_allclose(ux - F / 2, expect, rtol=1e-6)
+
+
+@pytest.mark.parametrize("omega,tol", [(1.0, 1e-3), (1.5, 2.5e-2)])
+def test_taylor_green_decay_rate(omega, tol):
+    """Linear-regime decay of a divergence-free Taylor-Green vortex: KE ~ exp(-2 nu |k|^2 t).
+    The BGK discretisation error is O(k^2) and vanishes at omega = 1."""
+    lat, (d, q) = 1, LATS[1]
+    n = 24
+    shp = (n, n, n)
+    k = 2 * np.pi / n
+    x = np.indices(shp) * k
+    u0 = 0.005
+    u = np.stack([u0 * np.sin(x[0]) * np.cos(x[1]) * np.cos(x[2]),
+                  -u0 * np.cos(x[0]) * np.sin(x[1]) * np.cos(x[2]), np.zeros(shp)])
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, np.ones(shp), u, f)
+    nu = (1 / omega - 0.5) / 3
+    c, _ = ref_lattice(lat)
+
+    def ke(ff):
+        r = ff.sum(0)
+        v = np.tensordot(c, ff, axes=(1, 0)) / r
+        return 0.5 * (v**2).sum()
+
+    t1, t2 = 50, 250
+    a = _run(lat, 0, f, omega, t1)
+    b = _run(lat, 0, a, omega, t2 - t1)
+    rate = np.log(ke(a) / ke(b)) / (t2 - t1)
+    expect = 2 * nu * 3 * k**2
+    assert abs(rate - expect) / expect < tol
+
+
+@pytest.mark.parametrize("lat", [0, 2])
+def test_kbc_relaxes_shear_like_bgk(lat):
+    """KBC (kbc.py:252-296) relaxes the shear part at omega whatever gamma is: the off-diagonal
+    second moments and the normal-stress differences of f_out - feq are (1 - omega) times those of
+    f - feq (at rho = 1); mass and momentum are conserved; at omega = 1 (beta = 1/2, gamma = 2) f_out = feq;
+    equilibrium input is a fixed point."""
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 5)
+    rng = np.random.default_rng(6)
+    c, w = ref_lattice(lat)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, 1 + 0.05 * rng.standard_normal(shp), 0.05 * rng.standard_normal((d,) + shp), f)
+    f += 2e-3 * rng.standard_normal(f.shape) * w.reshape(-1, 1, 1, 1)
+    # The reference scales the shear part by rho (delta_s = shear * rho), so the identity is
+    # exact at rho = 1: normalise every cell.
+    f /= f.sum(0)
+    rho = f.sum(0)
+    u = np.tensordot(c, f, axes=(1, 0)) / rho
+    feq = np.zeros_like(f)
+    orc.equilibrium(lat, orc.F64, 1, rho, u, feq)
+
+    def pi(g):
+        return np.einsum("al,bl,l...->ab...", c, c, g)
+
+    for omega in (0.8, 1.3, 1.9):
+        out = orc.collide(lat, 1, orc.F64, 1, f, feq, rho, u, omega)
+        np.testing.assert_allclose(out.sum(0), rho, atol=1e-14)
+        np.testing.assert_allclose(np.tensordot(c, out, axes=(1, 0)), rho * u, atol=1e-14)
+        p_in, p_out = pi(f - feq), pi(out - feq)
+        for a in range(d):
+            for b in range(d):
+                if a != b:
+                    np.testing.assert_allclose(p_out[a, b], (1 - omega) * p_in[a, b], atol=1e-14)
+        for a in range(d - 1):
+            na_in = p_in[a, a] - p_in[d - 1, d - 1]
+            na_out = p_out[a, a] - p_out[d - 1, d - 1]
+            np.testing.assert_allclose(na_out, (1 - omega) * na_in, atol=1e-14)
+    np.testing.assert_allclose(orc.collide(lat, 1, orc.F64, 1, f, feq, rho, u, 1.0), feq, atol=1e-15)
+    np.testing.assert_allclose(orc.collide(lat, 1, orc.F64, 1, feq, feq, rho, u, 1.6), feq, atol=1e-15)
+
+
+def test_momentum_exchange_symmetric_body_has_no_lift():
+    lat, (d, q) = 1, LATS[1]
+    n = 16
+    shp = (n, n, n)
+    ind = _sphere(n, 3)
+    idx = np.array(ind)
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.ones(idx.shape[1], np.uint8), 1, bc_mask, miss)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, np.ones(shp), np.stack([np.full(shp, 0.02), np.zeros(shp), np.zeros(shp)]), f)
+    out = _run(lat, 0, f, 1.2, 20, bc_mask, miss, [dict(id=1, kind=orc.HALFWAY_BB)])
+    F = orc.momentum_transfer(lat, orc.F64, 1, dict(id=1, kind=orc.HALFWAY_BB), out, bc_mask, miss)
+    assert F[0] > 0  # drag along the flow
+    assert abs(F[1]) < 1e-12 * abs(F[0]) + 1e-15 and abs(F[2]) < 1e-12 * abs(F[0]) + 1e-15

@@ -1,0 +1,276 @@
+// Shared definitions of the gfx950 LBM backend: lattice tables, population storage formats,
+// launch geometry and error reporting.  Everything here is header-only and used by every
+// translation unit of libxlbhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/xlb_hip.h"
+
+namespace xlbh {
+
+// ------------------------------------------------------------------------------------------
+// Error reporting: every C entry point returns 0 or a non-zero code and leaves a message in a
+// thread-local string read by xlbh_last_error().
+// ------------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+void clear_error();
+
+#define XLBH_REQUIRE(cond, msg)        \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::xlbh::set_error(msg);          \
+      return 1;                        \
+    }                                  \
+  } while (0)
+
+#define XLBH_HIP_CHECK(expr)                                                              \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      ::xlbh::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+      return 2;                                                                           \
+    }                                                                                     \
+  } while (0)
+
+#define XLBH_LAUNCH_CHECK() XLBH_HIP_CHECK(hipGetLastError())
+
+// ------------------------------------------------------------------------------------------
+// Lattice tables, built at compile time from the construction rules of
+// xlb/velocity_set/d2q9.py:16-24 (explicit list), d3q19.py:17-30 and d3q27.py:17-32
+// (itertools.product([0, -1, 1], repeat=3), D3Q19 keeping |c|_1 <= 2), with the opposite
+// index, the symmetric second-moment basis cc (order xx, xy, xz, yy, yz, zz / xx, xy, yy) and
+// qi = cc - cs^2 delta with doubled off-diagonals (velocity_set.py:124-164).
+// ------------------------------------------------------------------------------------------
+struct LatTables {
+  int d = 0, q = 0, nt = 0;
+  int c[3][27] = {};
+  double w[27] = {};
+  int opp[27] = {};
+  int cc[27][6] = {};
+  double qi[27][6] = {};
+  bool main[27] = {};  // |c|_1 == 1
+};
+
+constexpr LatTables make_tables(int lat) {
+  LatTables t{};
+  if (lat == XLBH_D2Q9) {
+    const int cx[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+    const int cy[9] = {0, 1, -1, 0, 1, -1, 0, 1, -1};
+    const double w[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36,
+                         1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 36};
+    t.d = 2;
+    t.q = 9;
+    for (int i = 0; i < 9; ++i) {
+      t.c[0][i] = cx[i];
+      t.c[1][i] = cy[i];
+      t.c[2][i] = 0;
+      t.w[i] = w[i];
+    }
+  } else {
+    const int v[3] = {0, -1, 1};
+    int n = 0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int e = 0; e < 3; ++e) {
+          const int s = (v[a] != 0) + (v[b] != 0) + (v[e] != 0);
+          if (lat == XLBH_D3Q19 && s > 2) continue;
+          t.c[0][n] = v[a];
+          t.c[1][n] = v[b];
+          t.c[2][n] = v[e];
+          if (lat == XLBH_D3Q19)
+            t.w[n] = s == 0 ? 1.0 / 3.0 : (s == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+          else
+            t.w[n] = s == 0 ? 8.0 / 27.0 : (s == 1 ? 2.0 / 27.0 : (s == 2 ? 1.0 / 54.0 : 1.0 / 216.0));
+          ++n;
+        }
+    t.d = 3;
+    t.q = n;
+  }
+  for (int i = 0; i < t.q; ++i) {
+    for (int j = 0; j < t.q; ++j)
+      if (t.c[0][j] == -t.c[0][i] && t.c[1][j] == -t.c[1][i] && t.c[2][j] == -t.c[2][i]) t.opp[i] = j;
+    const int s = (t.c[0][i] != 0) + (t.c[1][i] != 0) + (t.c[2][i] != 0);
+    t.main[i] = (s == 1);
+  }
+  t.nt = t.d * (t.d + 1) / 2;
+  for (int i = 0; i < t.q; ++i) {
+    int k = 0;
+    for (int a = 0; a < t.d; ++a)
+      for (int b = a; b < t.d; ++b) {
+        t.cc[i][k] = t.c[a][i] * t.c[b][i];
+        const bool diag = (a == b);
+        t.qi[i][k] = diag ? (double)t.cc[i][k] - 1.0 / 3.0 : 2.0 * (double)t.cc[i][k];
+        ++k;
+      }
+  }
+  return t;
+}
+
+template <int LAT>
+struct Lat {
+  static constexpr LatTables T = make_tables(LAT);
+  static constexpr int Q = T.q;
+  static constexpr int D = T.d;
+  static constexpr int NT = T.nt;
+};
+
+// ------------------------------------------------------------------------------------------
+// Population storage formats.  Loads convert to the compute type C (adding w_l back for the
+// shifted formats), stores round from C.  bf16 is kept as its raw 16 bits.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_bits_to_float(uint16_t b) {
+  return __uint_as_float(((uint32_t)b) << 16);
+}
+
+// Round-to-nearest-even float -> bf16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t float_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <int FMT>
+struct Fmt;
+template <>
+struct Fmt<XLBH_F32> {
+  using raw = float;
+  static constexpr bool shifted = false;
+};
+template <>
+struct Fmt<XLBH_F64> {
+  using raw = double;
+  static constexpr bool shifted = false;
+};
+template <>
+struct Fmt<XLBH_F16> {
+  using raw = _Float16;
+  static constexpr bool shifted = false;
+};
+template <>
+struct Fmt<XLBH_BF16> {
+  using raw = uint16_t;
+  static constexpr bool shifted = false;
+};
+template <>
+struct Fmt<XLBH_F16_SHIFTED> {
+  using raw = _Float16;
+  static constexpr bool shifted = true;
+};
+template <>
+struct Fmt<XLBH_BF16_SHIFTED> {
+  using raw = uint16_t;
+  static constexpr bool shifted = true;
+};
+
+template <class R, class C>
+__device__ __forceinline__ C raw_to(R v) {
+  if constexpr (sizeof(R) == 2 && !__is_same(R, _Float16))
+    return (C)bf16_bits_to_float(v);
+  else if constexpr (__is_same(R, _Float16))
+    return (C)(float)v;
+  else
+    return (C)v;
+}
+
+template <class R, class C>
+__device__ __forceinline__ R to_raw(C v) {
+  if constexpr (sizeof(R) == 2 && !__is_same(R, _Float16))
+    return float_to_bf16_bits((float)v);
+  else
+    return (R)v;
+}
+
+// Load population l (weight w) of format FMT as compute type C.
+template <int FMT, class C>
+__device__ __forceinline__ C ld_pop(const typename Fmt<FMT>::raw* p, double w) {
+  C v = raw_to<typename Fmt<FMT>::raw, C>(*p);
+  if constexpr (Fmt<FMT>::shifted) v += (C)w;
+  return v;
+}
+
+template <int FMT, class C>
+__device__ __forceinline__ void st_pop(typename Fmt<FMT>::raw* p, C v, double w) {
+  if constexpr (Fmt<FMT>::shifted) v -= (C)w;
+  *p = to_raw<typename Fmt<FMT>::raw, C>(v);
+}
+
+// Macroscopic-field element types (rho, u, Pi outputs of the standalone operators).
+template <int FT>
+struct FieldT;
+template <>
+struct FieldT<XLBH_FIELD_F32> {
+  using raw = float;
+};
+template <>
+struct FieldT<XLBH_FIELD_F64> {
+  using raw = double;
+};
+template <>
+struct FieldT<XLBH_FIELD_F16> {
+  using raw = _Float16;
+};
+template <>
+struct FieldT<XLBH_FIELD_BF16> {
+  using raw = uint16_t;
+};
+
+// ------------------------------------------------------------------------------------------
+// Launch geometry of the cell sweeps.  A 256-thread block covers a (by x bz) tile of the two
+// fastest axes; bz (a power of two, 64..256) runs along the unit-stride axis so a block reads
+// whole cache lines and keeps the +-1 shifted pulls of neighbouring lanes inside the block.
+// 3-D: fast = z, mid = y, slow = x.  2-D (nz == 1): fast = y, mid = x.
+// ------------------------------------------------------------------------------------------
+struct Sweep {
+  int nx, ny, nz;
+  int lbz;  // log2(bz)
+};
+
+inline Sweep make_sweep(int nx, int ny, int nz) {
+  const int fast = (nz > 1) ? nz : ny;
+  int lbz = 6;
+  while (lbz < 8 && (1 << lbz) < fast) ++lbz;
+  return Sweep{nx, ny, nz, lbz};
+}
+
+inline dim3 sweep_grid(const Sweep& s) {
+  const int bz = 1 << s.lbz, by = 256 / bz;
+  if (s.nz > 1) return dim3((s.nz + bz - 1) / bz, (s.ny + by - 1) / by, s.nx);
+  return dim3((s.ny + bz - 1) / bz, (s.nx + by - 1) / by, 1);
+}
+
+// Thread -> cell coordinates; false when the thread lies outside the grid.
+__device__ __forceinline__ bool sweep_cell(const Sweep& s, int& x, int& y, int& z) {
+  const int bz = 1 << s.lbz;
+  const int t = threadIdx.x;
+  const int fast = blockIdx.x * bz + (t & (bz - 1));
+  const int mid = blockIdx.y * (256 >> s.lbz) + (t >> s.lbz);
+  if (s.nz > 1) {
+    z = fast;
+    y = mid;
+    x = blockIdx.z;
+    return z < s.nz && y < s.ny;
+  }
+  z = 0;
+  y = fast;
+  x = mid;
+  return y < s.ny && x < s.nx;
+}
+
+__device__ __forceinline__ int wrap(int v, int n) {
+  v %= n;
+  return v < 0 ? v + n : v;
+}
+
+inline bool valid_lattice(int lat) { return lat == XLBH_D2Q9 || lat == XLBH_D3Q19 || lat == XLBH_D3Q27; }
+inline bool valid_store(int s) { return s >= XLBH_F32 && s <= XLBH_BF16_SHIFTED; }
+
+int check_grid(int lattice, int nx, int ny, int nz);
+
+}  // namespace xlbh

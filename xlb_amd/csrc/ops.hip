@@ -1,0 +1,439 @@
+// Standalone operators: the reference's per-operator Warp kernels (used on their own by the
+// operator API and its tests; the stepper uses the fused kernel of step.hpp instead).
+#include "physics.hpp"
+
+namespace xlbh {
+
+// ------------------------------------------------------------------------------------------
+// Stream: f_out[l](x) = f_in[l](x - c_l), periodic (stream.py:59-99).  A pure permutation of
+// stored values, so raw elements are moved without conversion.
+// ------------------------------------------------------------------------------------------
+template <int LAT, class R>
+__global__ __launch_bounds__(256) void k_stream(const R* __restrict__ fin, R* __restrict__ fout, Sweep sw,
+                                                uint32_t N) {
+  constexpr LatTables T = make_tables(LAT);
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const int nx = sw.nx, ny = sw.ny, nz = sw.nz;
+  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
+    fout[(size_t)l * N + cell] = fin[(size_t)l * N + ((uint32_t)sx * ny + sy) * nz + sz];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Equilibrium: f = feq(rho, u)  (quadratic_equilibrium.py:62-98).
+// ------------------------------------------------------------------------------------------
+template <int LAT, int FMT, class C, class FR>
+__global__ __launch_bounds__(256) void k_equilibrium(const FR* __restrict__ rho, const FR* __restrict__ u,
+                                                     typename Fmt<FMT>::raw* __restrict__ f, Sweep sw,
+                                                     uint32_t N) {
+  constexpr LatTables T = make_tables(LAT);
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
+  C uu[3] = {C(0), C(0), C(0)};
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) uu[d] = raw_to<FR, C>(u[(size_t)d * N + cell]);
+  const C r = raw_to<FR, C>(rho[cell]);
+  const C us = usqr15<LAT>(uu);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) st_pop<FMT, C>(f + (size_t)l * N + cell, feq_l<LAT>(l, r, uu, us), T.w[l]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Moments: which = 0 rho+u, 1 rho, 2 u given rho, 3 Pi (macroscopic.py:39-55,
+// zero_moment.py:29-42, first_moment.py:40-56, second_moment.py:81-98).
+// ------------------------------------------------------------------------------------------
+template <int LAT, int FMT, class C, class FR>
+__global__ __launch_bounds__(256) void k_moments(const typename Fmt<FMT>::raw* __restrict__ f, FR* rho,
+                                                 FR* u, FR* pi, int which, Sweep sw, uint32_t N) {
+  constexpr LatTables T = make_tables(LAT);
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
+  C ff[T.q];
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) ff[l] = ld_pop<FMT, C>(f + (size_t)l * N + cell, T.w[l]);
+  if (which == 3) {
+#pragma unroll
+    for (int t = 0; t < T.nt; ++t) {
+      C s = C(0);
+#pragma unroll
+      for (int l = 0; l < T.q; ++l) {
+        if (T.cc[l][t] == 1) s += ff[l];
+        else if (T.cc[l][t] == -1) s -= ff[l];
+      }
+      pi[(size_t)t * N + cell] = to_raw<FR, C>(s);
+    }
+    return;
+  }
+  C r, uu[3];
+  if (which == 2) {
+    r = raw_to<FR, C>(rho[cell]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uu[d] = C(0);
+#pragma unroll
+    for (int l = 0; l < T.q; ++l)
+#pragma unroll
+      for (int d = 0; d < T.d; ++d) {
+        if (T.c[d][l] == 1) uu[d] += ff[l];
+        else if (T.c[d][l] == -1) uu[d] -= ff[l];
+      }
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) uu[d] = uu[d] / r;
+  } else {
+    macroscopic<LAT>(ff, r, uu);
+  }
+  if (which == 0 || which == 1) rho[cell] = to_raw<FR, C>(r);
+  if (which == 0 || which == 2) {
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) u[(size_t)d * N + cell] = to_raw<FR, C>(uu[d]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Collision kernels: fout = collide(f, feq, rho, u, omega) (+ force).  BGK uses feq as given
+// (bgk.py:37-62); KBC reads rho, u and feq from the fields (kbc.py:299-329).
+// ------------------------------------------------------------------------------------------
+template <int LAT, int COLL, int FMT, class C, class FR>
+__global__ __launch_bounds__(256) void k_collide(const typename Fmt<FMT>::raw* __restrict__ f,
+                                                 const typename Fmt<FMT>::raw* __restrict__ feq,
+                                                 typename Fmt<FMT>::raw* __restrict__ fout,
+                                                 const FR* __restrict__ rho, const FR* __restrict__ u,
+                                                 C omega, int force_mode, double fu0, double fu1,
+                                                 double fu2, const C* __restrict__ force_field, Sweep sw,
+                                                 uint32_t N) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
+  C ff[Q], fe[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    ff[l] = ld_pop<FMT, C>(f + (size_t)l * N + cell, T.w[l]);
+    fe[l] = ld_pop<FMT, C>(feq + (size_t)l * N + cell, T.w[l]);
+  }
+  C out[Q];
+  C r = C(0), uu[3] = {C(0), C(0), C(0)};
+  const bool need_macro = (COLL == XLBH_KBC) || force_mode != XLBH_FORCE_NONE;
+  if (need_macro) {
+    r = raw_to<FR, C>(rho[cell]);
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) uu[d] = raw_to<FR, C>(u[(size_t)d * N + cell]);
+  }
+  if constexpr (COLL == XLBH_BGK) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) out[l] = ff[l] - omega * (ff[l] - fe[l]);
+  } else {
+    C fneq[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fneq[l] = ff[l] - fe[l];
+    C pi[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) pi[t] = C(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+#pragma unroll
+      for (int t = 0; t < T.nt; ++t) {
+        if (T.cc[l][t] == 1) pi[t] += fneq[l];
+        else if (T.cc[l][t] == -1) pi[t] -= fneq[l];
+      }
+    const C beta = C(0.5) * omega, inv_beta = C(1) / beta;
+    C sp1 = C(0), sp2 = C(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      C ds = kbc_shear<LAT>(l, pi) * r;
+      if constexpr (T.d == 2) ds = ds / C(4);
+      const C dh = fneq[l] - ds;
+      const C tmp = dh / fe[l];
+      sp1 += tmp * ds;
+      sp2 += tmp * dh;
+    }
+    const C gamma = inv_beta - (C(2) - inv_beta) * sp1 / (C(1e-32) + sp2);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      C ds = kbc_shear<LAT>(l, pi) * r;
+      if constexpr (T.d == 2) ds = ds / C(4);
+      const C dh = fneq[l] - ds;
+      out[l] = ff[l] - beta * (C(2) * ds + gamma * dh);
+    }
+  }
+  if (force_mode != XLBH_FORCE_NONE) {
+    C F[3] = {C(fu0), C(fu1), C(fu2)};
+    if (force_mode == XLBH_FORCE_FIELD) {
+#pragma unroll
+      for (int d = 0; d < T.d; ++d) F[d] = force_field[(size_t)d * N + cell];
+    }
+    C uf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uf[d] = uu[d] + F[d];
+    const C usf = usqr15<LAT>(uf);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) out[l] += feq_l<LAT>(l, r, uf, usf) - fe[l];
+  }
+#pragma unroll
+  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout + (size_t)l * N + cell, out[l], T.w[l]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Standalone boundary condition (BoundaryCondition._construct_kernel, boundary_condition.py:86-117):
+// at cells with bc_mask == id, f_post = functional(f_pre, f_post); elsewhere unchanged.
+// The functional of every kind is its streaming-step form (FullwayBB: flipped f_pre).
+// ------------------------------------------------------------------------------------------
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc desc,
+                                                  const typename Fmt<FMT>::raw* fpre,
+                                                  typename Fmt<FMT>::raw* fpost,
+                                                  const uint8_t* __restrict__ bc_mask,
+                                                  const uint32_t* __restrict__ missing, const C* aux,
+                                                  Sweep sw, uint32_t N) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
+  if (bc_mask[cell] != id) return;
+  const uint32_t mm = missing ? missing[cell] : 0u;
+  C f[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fpost + (size_t)l * N + cell, T.w[l]);
+  auto pre = [&](int l) { return ld_pop<FMT, C>(fpre + (size_t)l * N + cell, T.w[l]); };
+  if (kind == XLBH_BC_EQUILIBRIUM) {
+    const C rb = (C)desc.rho;
+    const C ub[3] = {(C)desc.u[0], (C)desc.u[1], (C)desc.u[2]};
+    const C us = usqr15<LAT>(ub);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) f[l] = feq_l<LAT>(l, rb, ub, us);
+  } else if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+      if (bit(mm, l)) f[l] = pre(T.opp[l]);
+  } else if (kind == XLBH_BC_FULLWAY_BB) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) f[l] = pre(T.opp[l]);
+  } else if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) f[l] = pre(l);
+  } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
+    const C pv = aux[cell];
+    zouhe_cell<LAT>(f, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
+                    kind >= XLBH_BC_REGULARIZED_VELOCITY);
+  }
+#pragma unroll
+  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fpost + (size_t)l * N + cell, f[l], T.w[l]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Host dispatch helpers.
+// ------------------------------------------------------------------------------------------
+template <class Fn>
+int with_lattice(int lat, Fn&& fn) {
+  switch (lat) {
+    case XLBH_D2Q9: return fn(std::integral_constant<int, XLBH_D2Q9>{});
+    case XLBH_D3Q19: return fn(std::integral_constant<int, XLBH_D3Q19>{});
+    case XLBH_D3Q27: return fn(std::integral_constant<int, XLBH_D3Q27>{});
+  }
+  set_error("unknown lattice id");
+  return 1;
+}
+
+// (store, compute) pairs accepted by the standalone operators: all stores with FP32 compute,
+// and F64/F32/F16 stores with FP64 compute.
+template <class Fn>
+int with_precision(int store, int compute, Fn&& fn) {
+  if (compute == XLBH_COMPUTE_F32) {
+    switch (store) {
+      case XLBH_F32: return fn(std::integral_constant<int, XLBH_F32>{}, float{});
+      case XLBH_F64: return fn(std::integral_constant<int, XLBH_F64>{}, float{});
+      case XLBH_F16: return fn(std::integral_constant<int, XLBH_F16>{}, float{});
+      case XLBH_BF16: return fn(std::integral_constant<int, XLBH_BF16>{}, float{});
+      case XLBH_F16_SHIFTED: return fn(std::integral_constant<int, XLBH_F16_SHIFTED>{}, float{});
+      case XLBH_BF16_SHIFTED: return fn(std::integral_constant<int, XLBH_BF16_SHIFTED>{}, float{});
+    }
+  } else if (compute == XLBH_COMPUTE_F64) {
+    switch (store) {
+      case XLBH_F64: return fn(std::integral_constant<int, XLBH_F64>{}, double{});
+      case XLBH_F32: return fn(std::integral_constant<int, XLBH_F32>{}, double{});
+      case XLBH_F16: return fn(std::integral_constant<int, XLBH_F16>{}, double{});
+    }
+  }
+  set_error("unsupported (store, compute) combination");
+  return 1;
+}
+
+template <class Fn>
+int with_field(int field, Fn&& fn) {
+  switch (field) {
+    case XLBH_FIELD_F32: return fn(float{});
+    case XLBH_FIELD_F64: return fn(double{});
+    case XLBH_FIELD_F16: return fn(_Float16{});
+    case XLBH_FIELD_BF16: return fn(uint16_t{});
+  }
+  set_error("unknown field dtype");
+  return 1;
+}
+
+}  // namespace xlbh
+
+using namespace xlbh;
+
+extern "C" {
+
+int xlbh_stream(int lattice, int store, const void* f_in, void* f_out, int nx, int ny, int nz,
+                void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(valid_store(store), "unknown store format");
+  XLBH_REQUIRE(f_in && f_out && f_in != f_out, "stream needs distinct non-null fields");
+  const Sweep sw = make_sweep(nx, ny, nz);
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  hipStream_t s = (hipStream_t)stream;
+  const bool two = (store != XLBH_F32 && store != XLBH_F64);
+  return with_lattice(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    if (store == XLBH_F64)
+      hipLaunchKernelGGL((k_stream<LAT, double>), sweep_grid(sw), dim3(256), 0, s, (const double*)f_in,
+                         (double*)f_out, sw, N);
+    else if (two)
+      hipLaunchKernelGGL((k_stream<LAT, uint16_t>), sweep_grid(sw), dim3(256), 0, s, (const uint16_t*)f_in,
+                         (uint16_t*)f_out, sw, N);
+    else
+      hipLaunchKernelGGL((k_stream<LAT, float>), sweep_grid(sw), dim3(256), 0, s, (const float*)f_in,
+                         (float*)f_out, sw, N);
+    XLBH_LAUNCH_CHECK();
+    return 0;
+  });
+}
+
+int xlbh_equilibrium(int lattice, int store, int compute, int field, const void* rho, const void* u,
+                     void* f, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(rho && u && f, "equilibrium needs rho, u and f");
+  const Sweep sw = make_sweep(nx, ny, nz);
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  hipStream_t s = (hipStream_t)stream;
+  return with_lattice(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    return with_precision(store, compute, [&](auto S, auto c) -> int {
+      constexpr int FMT = decltype(S)::value;
+      using C = decltype(c);
+      return with_field(field, [&](auto fr) -> int {
+        using FR = decltype(fr);
+        hipLaunchKernelGGL((k_equilibrium<LAT, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s, (const FR*)rho,
+                           (const FR*)u, (typename Fmt<FMT>::raw*)f, sw, N);
+        XLBH_LAUNCH_CHECK();
+        return 0;
+      });
+    });
+  });
+}
+
+int xlbh_moments(int lattice, int store, int compute, int field, int which, const void* f, void* rho,
+                 void* u, void* pi, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(which >= 0 && which <= 3, "moments: which must be 0..3");
+  XLBH_REQUIRE(f, "moments: f is null");
+  XLBH_REQUIRE(which == 3 || rho, "moments: rho is null");
+  XLBH_REQUIRE(!(which == 0 || which == 2) || u, "moments: u is null");
+  XLBH_REQUIRE(which != 3 || pi, "moments: pi is null");
+  const Sweep sw = make_sweep(nx, ny, nz);
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  hipStream_t s = (hipStream_t)stream;
+  return with_lattice(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    return with_precision(store, compute, [&](auto S, auto c) -> int {
+      constexpr int FMT = decltype(S)::value;
+      using C = decltype(c);
+      return with_field(field, [&](auto fr) -> int {
+        using FR = decltype(fr);
+        hipLaunchKernelGGL((k_moments<LAT, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
+                           (const typename Fmt<FMT>::raw*)f, (FR*)rho, (FR*)u, (FR*)pi, which, sw, N);
+        XLBH_LAUNCH_CHECK();
+        return 0;
+      });
+    });
+  });
+}
+
+int xlbh_collide(int lattice, int collision, int store, int compute, int field, const void* f,
+                 const void* feq, void* fout, const void* rho, const void* u, double omega, int force_mode,
+                 const double* force_uniform, const void* force_field, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(f && feq && fout, "collide needs f, feq and fout");
+  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC, "unknown collision");
+  XLBH_REQUIRE(!(collision == XLBH_KBC && lattice == XLBH_D3Q19),
+               "KBC collision is defined for D2Q9 and D3Q27 only (kbc.py:183-185)");
+  XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
+  XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
+  XLBH_REQUIRE(force_mode != XLBH_FORCE_FIELD || force_field, "force field is null");
+  XLBH_REQUIRE((collision == XLBH_BGK && force_mode == XLBH_FORCE_NONE) || (rho && u),
+               "KBC and forced collisions need rho and u");
+  const Sweep sw = make_sweep(nx, ny, nz);
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  hipStream_t s = (hipStream_t)stream;
+  double fu[3] = {0, 0, 0};
+  if (force_mode == XLBH_FORCE_UNIFORM)
+    for (int d = 0; d < 3; ++d) fu[d] = force_uniform[d];
+  return with_lattice(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    return with_precision(store, compute, [&](auto S, auto c) -> int {
+      constexpr int FMT = decltype(S)::value;
+      using C = decltype(c);
+      using R = typename Fmt<FMT>::raw;
+      return with_field(field, [&](auto fr) -> int {
+        using FR = decltype(fr);
+        if (collision == XLBH_BGK)
+          hipLaunchKernelGGL((k_collide<LAT, XLBH_BGK, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
+                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega,
+                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
+        else if constexpr (LAT != XLBH_D3Q19)
+          hipLaunchKernelGGL((k_collide<LAT, XLBH_KBC, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
+                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega,
+                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
+        XLBH_LAUNCH_CHECK();
+        return 0;
+      });
+    });
+  });
+}
+
+int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
+                  const void* f_pre, void* f_post, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                  const void* aux, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(f_pre && f_post && bc_mask, "apply_bc needs f_pre, f_post and bc_mask");
+  XLBH_REQUIRE(kind >= XLBH_BC_NONE && kind <= XLBH_BC_EXTRAPOLATION_OUTFLOW, "unknown BC kind");
+  XLBH_REQUIRE(bc_id >= 0 && bc_id <= 255, "bc_id out of range");
+  const bool needs_missing = kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW ||
+                             (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE);
+  XLBH_REQUIRE(!needs_missing || missing_bits, "this BC kind needs the missing bits");
+  XLBH_REQUIRE(!(kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) || aux,
+               "Zou-He / regularized BCs need the aux table");
+  xlbh_bc_desc d{};
+  if (desc) d = *desc;
+  const Sweep sw = make_sweep(nx, ny, nz);
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  hipStream_t s = (hipStream_t)stream;
+  return with_lattice(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    return with_precision(store, compute, [&](auto S, auto c) -> int {
+      constexpr int FMT = decltype(S)::value;
+      using C = decltype(c);
+      using R = typename Fmt<FMT>::raw;
+      hipLaunchKernelGGL((k_apply_bc<LAT, FMT, C>), sweep_grid(sw), dim3(256), 0, s, kind, bc_id, d,
+                         (const R*)f_pre, (R*)f_post, bc_mask, missing_bits, (const C*)aux, sw, N);
+      XLBH_LAUNCH_CHECK();
+      return 0;
+    });
+  });
+}
+
+}  // extern "C"

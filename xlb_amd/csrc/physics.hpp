@@ -1,0 +1,244 @@
+// Per-cell LBM physics on register-resident populations f[Q] (compute type C).
+// Each function states the reference routine whose arithmetic it restates; the order of the
+// floating-point accumulations follows the reference's loops so FP32 results agree to
+// rounding.
+#pragma once
+
+#include "common.hpp"
+
+namespace xlbh {
+
+// rho = sum_l f_l ; u_d = (sum_l c_dl f_l) / rho.
+// xlb/operator/macroscopic/zero_moment.py:22-27, first_moment.py:25-38.
+template <int LAT, class C>
+__device__ __forceinline__ void macroscopic(const C* f, C& rho, C* u) {
+  constexpr LatTables T = make_tables(LAT);
+  rho = C(0);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) rho += f[l];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u[d] = C(0);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) {
+      if (T.c[d][l] == 1) u[d] += f[l];
+      else if (T.c[d][l] == -1) u[d] -= f[l];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) u[d] = u[d] / rho;
+}
+
+// 1.5 |u|^2 (the usqr term of quadratic_equilibrium.py:52-53).
+template <int LAT, class C>
+__device__ __forceinline__ C usqr15(const C* u) {
+  constexpr LatTables T = make_tables(LAT);
+  C s = C(0);
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) s += u[d] * u[d];
+  return C(1.5) * s;
+}
+
+// feq_l = rho w_l (1 + cu (1 + cu/2) - 1.5|u|^2), cu = 3 c_l.u.
+// xlb/operator/equilibrium/quadratic_equilibrium.py:35-60.
+template <int LAT, class C>
+__device__ __forceinline__ C feq_l(int l, C rho, const C* u, C usqr) {
+  constexpr LatTables T = make_tables(LAT);
+  C cu = C(0);
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) {
+    if (T.c[d][l] == 1) cu += u[d];
+    else if (T.c[d][l] == -1) cu -= u[d];
+  }
+  cu *= C(3);
+  return rho * C(T.w[l]) * (C(1) + cu * (C(1) + C(0.5) * cu) - usqr);
+}
+
+// Second moment of (f - feq): Pi_t = sum_l cc_lt (f_l - feq_l).  second_moment.py:68-78.
+template <int LAT, class C>
+__device__ __forceinline__ void pi_neq(const C* f, C rho, const C* u, C usqr, C* pi) {
+  constexpr LatTables T = make_tables(LAT);
+#pragma unroll
+  for (int t = 0; t < 6; ++t) pi[t] = C(0);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    const C fneq = f[l] - feq_l<LAT>(l, rho, u, usqr);
+#pragma unroll
+    for (int t = 0; t < T.nt; ++t) {
+      if (T.cc[l][t] == 1) pi[t] += fneq;
+      else if (T.cc[l][t] == -1) pi[t] -= fneq;
+    }
+  }
+}
+
+// KBC shear part s_l of the non-equilibrium populations from Pi (kbc.py:192-250):
+// axis directions get the normal-stress differences, in-plane diagonals +-Pi_ab/4, the rest 0.
+template <int LAT, class C>
+__device__ __forceinline__ C kbc_shear(int l, const C* pi) {
+  constexpr LatTables T = make_tables(LAT);
+  const int ax = T.c[0][l], ay = T.c[1][l], az = T.c[2][l];
+  const int s = (ax != 0) + (ay != 0) + (az != 0);
+  if constexpr (T.d == 3) {
+    const C nxz = pi[0] - pi[5];
+    const C nyz = pi[3] - pi[5];
+    if (s == 1) {
+      if (ax != 0) return (C(2) * nxz - nyz) / C(6);
+      if (ay != 0) return (-nxz + C(2) * nyz) / C(6);
+      return (-nxz - nyz) / C(6);
+    }
+    if (s == 2) {
+      if (az == 0) return (ax * ay > 0) ? pi[1] / C(4) : -pi[1] / C(4);
+      if (ay == 0) return (ax * az > 0) ? pi[2] / C(4) : -pi[2] / C(4);
+      return (ay * az > 0) ? pi[4] / C(4) : -pi[4] / C(4);
+    }
+    return C(0);
+  } else {
+    const C n = pi[0] - pi[2];
+    if (s == 1) return (ax != 0) ? n : -n;
+    if (s == 2) return (ax * ay > 0) ? pi[1] : -pi[1];
+    return C(0);
+  }
+}
+
+// In-place collision f <- collide(f) with optional exact-difference body force F (delta u).
+//   BGK: f - omega (f - feq)                                      bgk.py:30-34
+//   KBC: f - beta (2 ds + gamma dh)                               kbc.py:268-296
+//   force: f += feq(rho, u + F) - feq(rho, u)                     exact_difference_force.py:71-86
+template <int LAT, int COLL, class C>
+__device__ __forceinline__ void collide(C* f, C omega, bool has_force, const C* F) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  C rho, u[3];
+  macroscopic<LAT>(f, rho, u);
+  const C usqr = usqr15<LAT>(u);
+  if constexpr (COLL == XLBH_BGK) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const C fe = feq_l<LAT>(l, rho, u, usqr);
+      f[l] = f[l] - omega * (f[l] - fe);
+    }
+  } else {
+    C pi[6];
+    pi_neq<LAT>(f, rho, u, usqr, pi);
+    const C beta = C(0.5) * omega;
+    const C inv_beta = C(1) / beta;
+    C sp1 = C(0), sp2 = C(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const C fe = feq_l<LAT>(l, rho, u, usqr);
+      const C fneq = f[l] - fe;
+      C ds = kbc_shear<LAT>(l, pi) * rho;
+      if constexpr (T.d == 2) ds = ds / C(4);
+      const C dh = fneq - ds;
+      const C tmp = dh / fe;
+      sp1 += tmp * ds;
+      sp2 += tmp * dh;
+    }
+    const C gamma = inv_beta - (C(2) - inv_beta) * sp1 / (C(1e-32) + sp2);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const C fe = feq_l<LAT>(l, rho, u, usqr);
+      const C fneq = f[l] - fe;
+      C ds = kbc_shear<LAT>(l, pi) * rho;
+      if constexpr (T.d == 2) ds = ds / C(4);
+      const C dh = fneq - ds;
+      f[l] = f[l] - beta * (C(2) * ds + gamma * dh);
+    }
+  }
+  if (has_force) {
+    C uf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uf[d] = u[d] + F[d];
+    const C usqr_f = usqr15<LAT>(uf);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const C fe = feq_l<LAT>(l, rho, u, usqr);
+      const C ff = feq_l<LAT>(l, rho, uf, usqr_f);
+      f[l] += ff - fe;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Boundary helpers (xlb/operator/boundary_condition/helper_functions_bc.py).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool bit(uint32_t m, int l) { return (m >> l) & 1u; }
+
+// Normal = -c of the first missing axis direction in lattice order (helper_functions_bc.py:75-86);
+// zero when no axis direction is missing.
+template <int LAT>
+__device__ __forceinline__ void bc_normal(uint32_t mm, int* n) {
+  constexpr LatTables T = make_tables(LAT);
+  n[0] = n[1] = n[2] = 0;
+#pragma unroll
+  for (int l = T.q - 1; l >= 0; --l) {
+    if (T.main[l] && bit(mm, l)) {
+      n[0] = -T.c[0][l];
+      n[1] = -T.c[1][l];
+      n[2] = -T.c[2][l];
+    }
+  }
+}
+
+// 2 * sum(known) + sum(middle) (helper_functions_bc.py:62-73).
+template <int LAT, class C>
+__device__ __forceinline__ C bc_fsum(const C* f, uint32_t mm) {
+  constexpr LatTables T = make_tables(LAT);
+  C known = C(0), middle = C(0);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    if (bit(mm, T.opp[l])) known += C(2) * f[l];
+    else if (!bit(mm, l)) middle += f[l];
+  }
+  return known + middle;
+}
+
+// Zou-He / regularized functional on post-stream populations f (in place), with the
+// prescribed value `pv` (normal speed or density).  bc_zouhe.py:279-338,
+// bc_regularized.py:134-196, helper_functions_bc.py:88-122.
+template <int LAT, class C>
+__device__ __forceinline__ void zouhe_cell(C* f, uint32_t mm, C pv, bool velocity, bool regularize) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int ni[3];
+  bc_normal<LAT>(mm, ni);
+  C n[3] = {C(ni[0]), C(ni[1]), C(ni[2])};
+  const C fsum = bc_fsum<LAT>(f, mm);
+  C rho, u[3];
+  if (velocity) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = -pv * n[d];
+    C un = C(0);
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) un += u[d] * n[d];
+    rho = fsum / (C(1) + un);
+  } else {
+    rho = pv;
+    const C un = -C(1) + fsum / rho;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = un * n[d];
+  }
+  const C usqr = usqr15<LAT>(u);
+  // Non-equilibrium bounce-back, sequential in l as in helper_functions_bc.py:88-97.
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    if (bit(mm, l)) {
+      const int o = T.opp[l];
+      f[l] = f[o] + feq_l<LAT>(l, rho, u, usqr) - feq_l<LAT>(o, rho, u, usqr);
+    }
+  }
+  if (regularize) {
+    C pi[6];
+    pi_neq<LAT>(f, rho, u, usqr, pi);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      C qp = C(0);
+#pragma unroll
+      for (int t = 0; t < T.nt; ++t) qp += C(T.qi[l][t]) * pi[t];
+      f[l] = feq_l<LAT>(l, rho, u, usqr) + C(4.5) * C(T.w[l]) * qp;
+    }
+  }
+}
+
+}  // namespace xlbh

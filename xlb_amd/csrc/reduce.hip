@@ -1,0 +1,233 @@
+// FP64 reductions over the lattice: momentum-exchange force and global mass/momentum.
+// Two passes, both in a fixed order, so results are bitwise reproducible run to run:
+// pass 1 reduces each 256-cell block (wave64 shuffle tree, then the 4 wave sums in LDS) into a
+// per-block partial; pass 2 has one block sum the partials in a fixed strided order + tree.
+#include "physics.hpp"
+
+namespace xlbh {
+
+constexpr int kRedWidth = 4;  // doubles per partial
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block of 256 threads -> out[0..W-1] written by thread 0.
+template <int W>
+__device__ __forceinline__ void block_sum(double (&v)[W], double* out) {
+  __shared__ double sh[4][W];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < W; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < W; ++k) sh[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < W; ++k) out[k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+}
+
+__global__ __launch_bounds__(256) void k_final_sum(const double* __restrict__ part, int nparts, int width,
+                                                   double* __restrict__ out) {
+  double v[kRedWidth] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nparts; i += 256)
+    for (int k = 0; k < width; ++k) v[k] += part[(size_t)i * kRedWidth + k];
+  double res[kRedWidth];
+  block_sum<kRedWidth>(v, res);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < width; ++k) out[k] = res[k];
+}
+
+// Momentum exchange (momentum_transfer.py:108-160): at cells with bc_mask == id whose rest
+// direction is not missing, re-pull the post-stream populations, apply the BC's streaming
+// functional, and sum over missing l of c_{opp l} (f_pc[opp l] + f_ps[l]).
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_momentum(const typename Fmt<FMT>::raw* __restrict__ fin,
+                                                  const uint8_t* __restrict__ bc_mask,
+                                                  const uint32_t* __restrict__ missing, const C* aux, int kind,
+                                                  int id, xlbh_bc_desc desc, int nx, int ny, int nz, uint32_t N,
+                                                  double* __restrict__ part) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  double m[kRedWidth] = {0, 0, 0, 0};
+  if (cell < N && bc_mask[cell] == id) {
+    const uint32_t mm = missing[cell];
+    if (!bit(mm, 0)) {
+      const int z = cell % nz;
+      const int y = (cell / nz) % ny;
+      const int x = cell / ((uint32_t)nz * ny);
+      C fpc[Q], fps[Q];
+#pragma unroll
+      for (int l = 0; l < Q; ++l) {
+        fpc[l] = ld_pop<FMT, C>(fin + (size_t)l * N + cell, T.w[l]);
+        const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
+        fps[l] = ld_pop<FMT, C>(fin + (size_t)l * N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
+      }
+      if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+          if (bit(mm, l)) fps[l] = fpc[T.opp[l]];
+      } else if (kind == XLBH_BC_FULLWAY_BB) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) fps[l] = fpc[T.opp[l]];
+      } else if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) fps[l] = fpc[l];
+      } else if (kind == XLBH_BC_EQUILIBRIUM) {
+        const C rb = (C)desc.rho;
+        const C ub[3] = {(C)desc.u[0], (C)desc.u[1], (C)desc.u[2]};
+        const C us = usqr15<LAT>(ub);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) fps[l] = feq_l<LAT>(l, rb, ub, us);
+      } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
+        zouhe_cell<LAT>(fps, mm, aux[cell], kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
+                        kind >= XLBH_BC_REGULARIZED_VELOCITY);
+      }
+#pragma unroll
+      for (int d = 0; d < T.d; ++d) {
+        C md = C(0);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+          if (bit(mm, l)) {
+            const C phi = fpc[T.opp[l]] + fps[l];
+            if (T.c[d][T.opp[l]] == 1) md += phi;
+            else if (T.c[d][T.opp[l]] == -1) md -= phi;
+          }
+        }
+        m[d] = (double)md;
+      }
+    }
+  }
+  block_sum<kRedWidth>(m, part + (size_t)blockIdx.x * kRedWidth);
+}
+
+// sum rho and sum c f over non-solid cells, accumulated in FP64 from the stored populations.
+template <int LAT, int FMT>
+__global__ __launch_bounds__(256) void k_mass_momentum(const typename Fmt<FMT>::raw* __restrict__ fin,
+                                                       const uint8_t* __restrict__ bc_mask, uint32_t N,
+                                                       double* __restrict__ part) {
+  constexpr LatTables T = make_tables(LAT);
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  double v[kRedWidth] = {0, 0, 0, 0};
+  if (cell < N && (bc_mask == nullptr || bc_mask[cell] != 255)) {
+#pragma unroll
+    for (int l = 0; l < T.q; ++l) {
+      const double f = ld_pop<FMT, double>(fin + (size_t)l * N + cell, T.w[l]);
+      v[0] += f;
+#pragma unroll
+      for (int d = 0; d < T.d; ++d) v[1 + d] += (double)T.c[d][l] * f;
+    }
+  }
+  block_sum<kRedWidth>(v, part + (size_t)blockIdx.x * kRedWidth);
+}
+
+}  // namespace xlbh
+
+using namespace xlbh;
+
+namespace {
+template <class Fn>
+int lat_dispatch(int lat, Fn&& fn) {
+  switch (lat) {
+    case XLBH_D2Q9: return fn(std::integral_constant<int, XLBH_D2Q9>{});
+    case XLBH_D3Q19: return fn(std::integral_constant<int, XLBH_D3Q19>{});
+    case XLBH_D3Q27: return fn(std::integral_constant<int, XLBH_D3Q27>{});
+  }
+  set_error("unknown lattice id");
+  return 1;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t xlbh_reduce_workspace_bytes(int64_t n_cells) {
+  return ((n_cells + 255) / 256) * kRedWidth * (int64_t)sizeof(double);
+}
+
+int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
+                           const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                           double* out, void* workspace, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(f && bc_mask && missing_bits && out && workspace, "momentum_transfer: null argument");
+  XLBH_REQUIRE(bc_id >= 1 && bc_id <= 254, "momentum_transfer: bc_id must be 1..254");
+  XLBH_REQUIRE(!(kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) || aux,
+               "momentum_transfer: Zou-He / regularized BCs need the aux table");
+  xlbh_bc_desc d{};
+  if (desc) d = *desc;
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  const int nb = (int)((N + 255) / 256);
+  double* part = static_cast<double*>(workspace);
+  hipStream_t s = (hipStream_t)stream;
+  return lat_dispatch(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    auto go = [&](auto S, auto c) -> int {
+      constexpr int FMT = decltype(S)::value;
+      using C = decltype(c);
+      hipLaunchKernelGGL((k_momentum<LAT, FMT, C>), dim3(nb), dim3(256), 0, s,
+                         (const typename Fmt<FMT>::raw*)f, bc_mask, missing_bits, (const C*)aux, kind, bc_id, d,
+                         nx, ny, nz, N, part);
+      XLBH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, part, nb, Lat<LAT>::D, out);
+      XLBH_LAUNCH_CHECK();
+      return 0;
+    };
+    if (compute == XLBH_COMPUTE_F32) {
+      switch (store) {
+        case XLBH_F32: return go(std::integral_constant<int, XLBH_F32>{}, float{});
+        case XLBH_F16: return go(std::integral_constant<int, XLBH_F16>{}, float{});
+        case XLBH_BF16: return go(std::integral_constant<int, XLBH_BF16>{}, float{});
+        case XLBH_F16_SHIFTED: return go(std::integral_constant<int, XLBH_F16_SHIFTED>{}, float{});
+        case XLBH_BF16_SHIFTED: return go(std::integral_constant<int, XLBH_BF16_SHIFTED>{}, float{});
+      }
+    } else if (compute == XLBH_COMPUTE_F64) {
+      switch (store) {
+        case XLBH_F64: return go(std::integral_constant<int, XLBH_F64>{}, double{});
+        case XLBH_F32: return go(std::integral_constant<int, XLBH_F32>{}, double{});
+        case XLBH_F16: return go(std::integral_constant<int, XLBH_F16>{}, double{});
+      }
+    }
+    set_error("unsupported (store, compute) combination");
+    return 1;
+  });
+}
+
+int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask, double* out, void* workspace,
+                       int nx, int ny, int nz, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(f && out && workspace, "mass_momentum: null argument");
+  XLBH_REQUIRE(valid_store(store), "unknown store format");
+  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  const int nb = (int)((N + 255) / 256);
+  double* part = static_cast<double*>(workspace);
+  hipStream_t s = (hipStream_t)stream;
+  return lat_dispatch(lattice, [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    auto go = [&](auto S) -> int {
+      constexpr int FMT = decltype(S)::value;
+      hipLaunchKernelGGL((k_mass_momentum<LAT, FMT>), dim3(nb), dim3(256), 0, s,
+                         (const typename Fmt<FMT>::raw*)f, bc_mask, N, part);
+      XLBH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, part, nb, 1 + Lat<LAT>::D, out);
+      XLBH_LAUNCH_CHECK();
+      return 0;
+    };
+    switch (store) {
+      case XLBH_F32: return go(std::integral_constant<int, XLBH_F32>{});
+      case XLBH_F64: return go(std::integral_constant<int, XLBH_F64>{});
+      case XLBH_F16: return go(std::integral_constant<int, XLBH_F16>{});
+      case XLBH_BF16: return go(std::integral_constant<int, XLBH_BF16>{});
+      case XLBH_F16_SHIFTED: return go(std::integral_constant<int, XLBH_F16_SHIFTED>{});
+      case XLBH_BF16_SHIFTED: return go(std::integral_constant<int, XLBH_BF16_SHIFTED>{});
+    }
+    set_error("unknown store format");
+    return 1;
+  });
+}
+
+}  // extern "C"

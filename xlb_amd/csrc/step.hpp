@@ -1,0 +1,237 @@
+// The fused pull-scheme LBM step: one thread per cell, one launch per time step.
+//
+// Restates IncompressibleNavierStokesStepper._construct_warp.kernel
+// (xlb/operator/stepper/nse_stepper.py:344-383) for gfx950:
+//   pull stream (periodic wrap)  -> streaming-step BC -> macroscopic -> equilibrium ->
+//   BGK/KBC collision (+ exact-difference force) -> collision-step BC ->
+//   extrapolation-outflow post-collision update -> store.
+//
+// Differences in mechanism (results identical on the output field):
+//  * Fluid cells (id 0) read exactly Q pulled populations and 1 byte of boundary id and write
+//    Q populations: 2*Q*S + 1 bytes per update.  The reference also reads the cell's own f_0
+//    and f_1 populations and Q bool mask bytes at every cell (nse_stepper.py:296-316).
+//  * Boundary cells read the packed missing bits (4 bytes) and, where the BC needs them, the
+//    own-cell pre-stream populations on demand.
+//  * Prescribed values of Zou-He / regularized BCs come from a side table (aux) instead of
+//    f_1[0] plus recovery writes into the input field (nse_stepper.py:318-342), so the input
+//    field is never written (no read/write race on f_in).
+#pragma once
+
+#include "physics.hpp"
+
+namespace xlbh {
+
+struct StepParams {
+  const void* f_in;
+  void* f_out;
+  const uint8_t* bc_mask;
+  const uint32_t* missing;
+  const void* aux;
+  const xlbh_bc_desc* table;
+  const void* force_field;
+  double fu[3];
+  int force_mode;
+  double omega;
+  Sweep sw;
+  uint32_t N;
+};
+
+template <int LAT, int COLL, int FMT, class C>
+__global__ __launch_bounds__(256) void k_step(StepParams p) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  using R = typename Fmt<FMT>::raw;
+
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+  const uint32_t N = p.N;
+  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+
+  int id = 0;
+  if (p.bc_mask != nullptr) {
+    id = p.bc_mask[cell];
+    if (id == 255) return;  // solid: no update (nse_stepper.py:356-358)
+  }
+
+  const R* __restrict__ fin = static_cast<const R*>(p.f_in);
+  R* __restrict__ fout = static_cast<R*>(p.f_out);
+
+  // Pull sources x - c for c in {-1, 0, +1} per axis, with periodic wrap (stream.py:64-81).
+  const uint32_t plane = (uint32_t)ny * nz;
+  uint32_t xo[3], yo[3], zo[3];
+  xo[0] = (uint32_t)(x == nx - 1 ? 0 : x + 1) * plane;
+  xo[1] = (uint32_t)x * plane;
+  xo[2] = (uint32_t)(x == 0 ? nx - 1 : x - 1) * plane;
+  yo[0] = (uint32_t)(y == ny - 1 ? 0 : y + 1) * nz;
+  yo[1] = (uint32_t)y * nz;
+  yo[2] = (uint32_t)(y == 0 ? ny - 1 : y - 1) * nz;
+  zo[0] = (uint32_t)(z == nz - 1 ? 0 : z + 1);
+  zo[1] = (uint32_t)z;
+  zo[2] = (uint32_t)(z == 0 ? nz - 1 : z - 1);
+
+  C f[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    const uint32_t src = xo[T.c[0][l] + 1] + yo[T.c[1][l] + 1] + zo[T.c[2][l] + 1];
+    f[l] = ld_pop<FMT, C>(fin + (size_t)l * N + src, T.w[l]);
+  }
+
+  int kind = XLBH_BC_NONE;
+  uint32_t mm = 0;
+  if (id != 0) {
+    const xlbh_bc_desc* d = p.table + id;
+    kind = d->kind;
+    mm = p.missing[cell];
+    // ---- streaming-step boundary conditions (apply_bc with is_post_streaming) ----
+    if (kind == XLBH_BC_EQUILIBRIUM) {
+      const C rb = (C)d->rho;
+      const C ub[3] = {(C)d->u[0], (C)d->u[1], (C)d->u[2]};
+      const C us = usqr15<LAT>(ub);
+#pragma unroll
+      for (int l = 0; l < Q; ++l) f[l] = feq_l<LAT>(l, rb, ub, us);
+    } else if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+      for (int l = 0; l < Q; ++l)
+        if (bit(mm, l)) f[l] = ld_pop<FMT, C>(fin + (size_t)T.opp[l] * N + cell, T.w[T.opp[l]]);
+    } else if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+      for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fin + (size_t)l * N + cell, T.w[l]);
+    } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
+      const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
+      const bool vel = (kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY);
+      const bool reg = (kind >= XLBH_BC_REGULARIZED_VELOCITY);
+      zouhe_cell<LAT>(f, mm, pv, vel, reg);
+    }
+  }
+
+  if (kind == XLBH_BC_FULLWAY_BB) {
+    // Collision-step full-way bounce-back: post-collision = flipped post-stream
+    // (bc_fullway_bounce_back.py:59-72); the collision result is discarded, so skip it.
+    C g[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) g[l] = f[T.opp[l]];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) f[l] = g[l];
+  } else {
+    C F[3] = {C(0), C(0), C(0)};
+    const bool has_force = p.force_mode != XLBH_FORCE_NONE;
+    if (p.force_mode == XLBH_FORCE_UNIFORM) {
+      F[0] = (C)p.fu[0];
+      F[1] = (C)p.fu[1];
+      F[2] = (C)p.fu[2];
+    } else if (p.force_mode == XLBH_FORCE_FIELD) {
+      const C* ff = static_cast<const C*>(p.force_field);
+#pragma unroll
+      for (int dd = 0; dd < T.d; ++dd) F[dd] = ff[(size_t)dd * N + cell];
+    }
+    collide<LAT, COLL>(f, (C)p.omega, has_force, F);
+  }
+
+  if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+    // Post-collision auxiliary update (bc_extrapolation_outflow.py:163-195): the outgoing slot
+    // opp(l) of every missing l receives (1 - cs) f_ps[l] + cs f_0[l](x - c_l - n).  The
+    // post-stream value f_ps[l] of a missing l is the own-cell pre-stream f[opp l].
+    // Out-of-domain sources wrap periodically (the reference reads them unchecked).
+    int n[3];
+    bc_normal<LAT>(mm, n);
+    const C cs = C(0.57735026918962584);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      if (bit(mm, l)) {
+        const int sx = wrap(x - (T.c[0][l] + n[0]), nx);
+        const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
+        const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
+        const uint32_t src = ((uint32_t)sx * ny + sy) * nz + sz;
+        const C faux = ld_pop<FMT, C>(fin + (size_t)l * N + src, T.w[l]);
+        const C fps = ld_pop<FMT, C>(fin + (size_t)T.opp[l] * N + cell, T.w[T.opp[l]]);
+        f[T.opp[l]] = (C(1) - cs) * fps + cs * faux;
+      }
+    }
+  }
+
+#pragma unroll
+  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout + (size_t)l * N + cell, f[l], T.w[l]);
+}
+
+// Host launcher, instantiated per lattice in step_<lattice>.hip.
+struct StepCall {
+  int collision, store, compute;
+  const void* f_in;
+  void* f_out;
+  const uint8_t* bc_mask;
+  const uint32_t* missing;
+  const void* aux;
+  const xlbh_bc_desc* table;
+  int force_mode;
+  const double* force_uniform;
+  const void* force_field;
+  int nx, ny, nz;
+  double omega;
+  hipStream_t stream;
+};
+
+template <int LAT, int COLL, int FMT, class C>
+int launch_step_t(const StepCall& c) {
+  StepParams p;
+  p.f_in = c.f_in;
+  p.f_out = c.f_out;
+  p.bc_mask = c.bc_mask;
+  p.missing = c.missing;
+  p.aux = c.aux;
+  p.table = c.table;
+  p.force_field = c.force_field;
+  p.force_mode = c.force_mode;
+  for (int d = 0; d < 3; ++d) p.fu[d] = (c.force_mode == XLBH_FORCE_UNIFORM) ? c.force_uniform[d] : 0.0;
+  p.omega = c.omega;
+  p.sw = make_sweep(c.nx, c.ny, c.nz);
+  p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
+  hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C>), sweep_grid(p.sw), dim3(256), 0, c.stream, p);
+  XLBH_LAUNCH_CHECK();
+  return 0;
+}
+
+// Dispatch over (collision, store, compute) for one lattice.  Valid precision pairs are the
+// reference's policies plus the BF16 and shifted 16-bit stores with FP32 compute.
+template <int LAT>
+int launch_step_lat(const StepCall& c) {
+  constexpr bool has_kbc = (LAT != XLBH_D3Q19);
+  if (c.collision == XLBH_KBC && !has_kbc) {
+    set_error("KBC collision is defined for D2Q9 and D3Q27 only (kbc.py:183-185)");
+    return 1;
+  }
+#define XLBH_STEP_CASES(COLL)                                                                   \
+  if (c.compute == XLBH_COMPUTE_F32) {                                                          \
+    switch (c.store) {                                                                          \
+      case XLBH_F32: return launch_step_t<LAT, COLL, XLBH_F32, float>(c);                       \
+      case XLBH_F16: return launch_step_t<LAT, COLL, XLBH_F16, float>(c);                       \
+      case XLBH_BF16: return launch_step_t<LAT, COLL, XLBH_BF16, float>(c);                     \
+      case XLBH_F16_SHIFTED: return launch_step_t<LAT, COLL, XLBH_F16_SHIFTED, float>(c);       \
+      case XLBH_BF16_SHIFTED: return launch_step_t<LAT, COLL, XLBH_BF16_SHIFTED, float>(c);     \
+      default: break;                                                                           \
+    }                                                                                           \
+  } else if (c.compute == XLBH_COMPUTE_F64) {                                                   \
+    switch (c.store) {                                                                          \
+      case XLBH_F64: return launch_step_t<LAT, COLL, XLBH_F64, double>(c);                      \
+      case XLBH_F32: return launch_step_t<LAT, COLL, XLBH_F32, double>(c);                      \
+      case XLBH_F16: return launch_step_t<LAT, COLL, XLBH_F16, double>(c);                      \
+      default: break;                                                                           \
+    }                                                                                           \
+  }
+  if (c.collision == XLBH_BGK) {
+    XLBH_STEP_CASES(XLBH_BGK)
+  } else if constexpr (has_kbc) {
+    if (c.collision == XLBH_KBC) {
+      XLBH_STEP_CASES(XLBH_KBC)
+    }
+  }
+#undef XLBH_STEP_CASES
+  set_error("unsupported (collision, store, compute) combination for the fused step");
+  return 1;
+}
+
+int launch_step_d2q9(const StepCall& c);
+int launch_step_d3q19(const StepCall& c);
+int launch_step_d3q27(const StepCall& c);
+
+}  // namespace xlbh

@@ -1,0 +1,7 @@
+// Fused-step instantiations for one lattice (kept in its own translation unit so the
+// lattices compile in parallel).
+#include "step.hpp"
+
+namespace xlbh {
+int launch_step_d3q19(const StepCall& c) { return launch_step_lat<XLBH_D3Q19>(c); }
+}  // namespace xlbh

@@ -1,0 +1,228 @@
+"""IncompressibleNavierStokesStepper (mirrors xlb/operator/stepper/nse_stepper.py:25-392 and
+stepper.py:6-34).
+
+``__call__(f_0, f_1, bc_mask, missing_mask, omega, timestep)`` runs ONE fused HIP kernel
+(libxlbhip ``xlbh_step``): pull streaming, streaming-step BCs, macroscopic, equilibrium,
+BGK/KBC collision (+ exact-difference force), collision-step BCs, extrapolation-outflow update
+and store, writing f_1 and returning (f_0, f_1); the caller swaps, as with the reference.
+
+Per-mask device state (derived once and re-derived only when the mask tensors change):
+packed missing bits (N uint32), the 256-entry BC table indexed by boundary id, and the aux side
+table of prescribed values.  ``run`` moves the caller's step/swap loop below the C ABI
+(``xlbh_run``).
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from xlb_amd import _lib
+from xlb_amd.compute_backend import ComputeBackend
+from xlb_amd.default_config import DefaultConfig
+from xlb_amd.helper.check_boundary_overlaps import check_bc_overlaps
+from xlb_amd.helper.nse_solver import create_nse_fields
+from xlb_amd.operator.boundary_condition.boundary_condition import ImplementationStep
+from xlb_amd.operator.boundary_masker import IndicesBoundaryMasker
+from xlb_amd.operator.boundary_masker.packing import packed_missing
+from xlb_amd.operator.collision import BGK, KBC, ForcedCollision, SmagorinskyLESBGK
+from xlb_amd.operator.equilibrium import QuadraticEquilibrium
+from xlb_amd.operator.macroscopic import Macroscopic
+from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.stream import Stream
+
+
+class Stepper(Operator):
+    """Base of steppers (stepper.py:6-34)."""
+
+    def __init__(self, grid, boundary_conditions):
+        self.grid = grid
+        self.boundary_conditions = boundary_conditions
+        super().__init__(DefaultConfig.velocity_set, DefaultConfig.default_precision_policy, DefaultConfig.default_backend)
+
+    def prepare_fields(self, initializer=None):
+        raise NotImplementedError("Subclasses must implement prepare_fields()")
+
+
+class IncompressibleNavierStokesStepper(Stepper):
+    def __init__(
+        self,
+        grid,
+        boundary_conditions=[],
+        collision_type="BGK",
+        streaming_scheme="pull",
+        forcing_scheme="exact_difference",
+        force_vector=None,
+        force_field=None,
+    ):
+        super().__init__(grid, list(boundary_conditions))
+        if collision_type == "BGK":
+            self.collision = BGK(self.velocity_set, self.precision_policy, self.compute_backend)
+        elif collision_type == "KBC":
+            self.collision = KBC(self.velocity_set, self.precision_policy, self.compute_backend)
+        elif collision_type == "SmagorinskyLESBGK":
+            self.collision = SmagorinskyLESBGK(self.velocity_set, self.precision_policy, self.compute_backend)
+        else:
+            raise ValueError(f"Unknown collision type {collision_type}")
+        self.collision_type = collision_type
+        if force_vector is not None or force_field is not None:
+            self.collision = ForcedCollision(
+                collision_operator=self.collision, forcing_scheme=forcing_scheme, force_vector=force_vector, force_field=force_field
+            )
+        self.streaming_scheme = streaming_scheme
+        assert streaming_scheme == "pull", f"Unknown or unimplemented streaming scheme for backend: {self.compute_backend}"
+        self.stream = Stream(self.velocity_set, self.precision_policy, self.compute_backend)
+        self.equilibrium = QuadraticEquilibrium(self.velocity_set, self.precision_policy, self.compute_backend)
+        self.macroscopic = Macroscopic(self.velocity_set, self.precision_policy, self.compute_backend)
+        for bc in self.boundary_conditions:
+            if bc.mesh_vertices is not None:
+                raise NotImplementedError("mesh-based boundary conditions (MeshBoundaryMasker) are not implemented by the MI355X backend")
+        self._state_key = None
+        self._bits = None
+        self._table = None
+        self._aux = None
+        self._has_bc_cells = True
+
+    # ------------------------------------------------------------------ fields
+    def prepare_fields(self, initializer=None):
+        """Allocate and initialise (f_0, f_1, bc_mask, missing_mask) (nse_stepper.py:58-98)."""
+        _, f_0, f_1, missing_mask, bc_mask = create_nse_fields(
+            grid=self.grid, velocity_set=self.velocity_set, compute_backend=self.compute_backend, precision_policy=self.precision_policy
+        )
+        if initializer is not None:
+            f_0 = initializer(self.grid, self.velocity_set, self.precision_policy, self.compute_backend)
+        else:
+            from xlb_amd.helper.initializers import initialize_eq
+
+            f_0 = initialize_eq(f_0, self.grid, self.velocity_set, self.precision_policy, self.compute_backend)
+        f_1.copy_(f_0)
+        bc_mask, missing_mask = self._process_boundary_conditions(self.boundary_conditions, bc_mask, missing_mask)
+        f_0, f_1 = self._initialize_auxiliary_data(self.boundary_conditions, f_0, f_1, bc_mask, missing_mask)
+        self._sync_state(bc_mask, missing_mask)
+        return f_0, f_1, bc_mask, missing_mask
+
+    @classmethod
+    def _process_boundary_conditions(cls, boundary_conditions, bc_mask, missing_mask):
+        """Overlap check and indices masker (nse_stepper.py:100-127)."""
+        check_bc_overlaps(boundary_conditions, DefaultConfig.velocity_set.d, DefaultConfig.default_backend)
+        indices_masker = IndicesBoundaryMasker(
+            velocity_set=DefaultConfig.velocity_set,
+            precision_policy=DefaultConfig.default_precision_policy,
+            compute_backend=DefaultConfig.default_backend,
+        )
+        bc_with_indices = [bc for bc in boundary_conditions if bc.indices is not None]
+        if bc_with_indices:
+            bc_mask, missing_mask = indices_masker(bc_with_indices, bc_mask, missing_mask)
+        return bc_mask, missing_mask
+
+    def _initialize_auxiliary_data(self, boundary_conditions, f_0, f_1, bc_mask, missing_mask):
+        """Fill the shared aux side table for BCs that carry prescribed values (nse_stepper.py:129-135)."""
+        aux_bcs = [bc for bc in boundary_conditions if bc.needs_aux_init]
+        if not aux_bcs:
+            self._aux = None
+            return f_0, f_1
+        nx, ny, nz = field_extents(bc_mask)
+        self._aux = torch.zeros(nx * ny * nz, dtype=self.compute_dtype, device=bc_mask.device)
+        for bc in aux_bcs:
+            bc.aux_data_init(f_0, f_1, bc_mask, missing_mask, aux=self._aux)
+        return f_0, f_1
+
+    # ------------------------------------------------------------------ device state
+    def _mask_key(self, bc_mask, missing_mask):
+        return (id(bc_mask), bc_mask._version, bc_mask.data_ptr(), id(missing_mask), missing_mask._version, missing_mask.data_ptr())
+
+    def _build_table(self, device):
+        descs = (_lib.BcDesc * 256)()
+        for bc in self.boundary_conditions:
+            if not (1 <= int(bc.id) <= 254):
+                raise ValueError(f"boundary id {bc.id} does not fit the uint8 bc_mask (1..254)")
+            descs[int(bc.id)] = bc.bc_desc()
+        raw = np.frombuffer(bytes(descs), dtype=np.uint8).copy()
+        return torch.from_numpy(raw).to(device)
+
+    def _sync_state(self, bc_mask, missing_mask):
+        key = self._mask_key(bc_mask, missing_mask)
+        if key == self._state_key:
+            return
+        _lib.require_cuda(bc_mask, missing_mask)
+        ext = field_extents(bc_mask)
+        expect_field(bc_mask, 1, ext, "bc_mask", torch.uint8)
+        expect_field(missing_mask, self.velocity_set.q, ext, "missing_mask", torch.bool)
+        self._bits = packed_missing(missing_mask, self.lattice_id)
+        self._table = self._build_table(bc_mask.device)
+        self._has_bc_cells = bool(bc_mask.any().item())
+        nx, ny, nz = field_extents(bc_mask)
+        aux_bcs = [bc for bc in self.boundary_conditions if bc.needs_aux_init]
+        if aux_bcs and (self._aux is None or self._aux.numel() != nx * ny * nz or self._state_key is not None):
+            # masks changed after prepare_fields (or were built by the caller): re-derive aux
+            self._aux = torch.zeros(nx * ny * nz, dtype=self.compute_dtype, device=bc_mask.device)
+            for bc in aux_bcs:
+                bc.aux_data_init(None, None, bc_mask, missing_mask, aux=self._aux)
+        self._state_key = self._mask_key(bc_mask, missing_mask)
+
+    def _force_args(self):
+        col = self.collision
+        if isinstance(col, ForcedCollision):
+            if col.force_field is not None:
+                ff = col.force_field
+                if ff.dtype != self.compute_dtype:
+                    ff = ff.to(self.compute_dtype)
+                    col.force_field = ff
+                return _lib.FORCE_FIELD, None, ff
+            return _lib.FORCE_UNIFORM, col.force_vector, None
+        return _lib.FORCE_NONE, None, None
+
+    def _collision_id(self):
+        return _lib.KBC if self.collision_type == "KBC" else _lib.BGK
+
+    def _launch_args(self, f_0, f_1, bc_mask, missing_mask):
+        nx, ny, nz = field_extents(f_0)
+        q, ext = self.velocity_set.q, (nx, ny, nz)
+        expect_field(f_0, q, ext, "f_0", self.store_dtype)
+        expect_field(f_1, q, ext, "f_1", self.store_dtype)
+        if f_0.data_ptr() == f_1.data_ptr():
+            raise ValueError("f_0 and f_1 must be distinct buffers")
+        expect_field(bc_mask, 1, ext, "bc_mask", torch.uint8)
+        self._sync_state(bc_mask, missing_mask)
+        force_mode, force_vec, force_field = self._force_args()
+        if force_field is not None:
+            expect_field(force_field, self.velocity_set.d, ext, "force_field", self.compute_dtype)
+        fu_ptr, keep = _lib.doubles(force_vec if force_vec is not None else (0.0, 0.0, 0.0))
+        if self._has_bc_cells:
+            masks = (_lib.ptr(bc_mask), _lib.ptr(self._bits), _lib.ptr(self._aux), _lib.ptr(self._table))
+        else:
+            masks = (None, None, None, None)
+        return nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field
+
+    # ------------------------------------------------------------------ stepping
+    @Operator.register_backend(ComputeBackend.MI355X)
+    def mi355x_implementation(self, f_0, f_1, bc_mask, missing_mask, omega, timestep):
+        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
+        _lib.check(
+            _lib.lib().xlbh_step(
+                self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
+                _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
+                nx, ny, nz, float(omega), int(timestep), _lib.stream_of(f_0),
+            ),
+            "IncompressibleNavierStokesStepper",
+        )
+        _lib.mark_written(f_1)
+        return f_0, f_1
+
+    def run(self, f_0, f_1, bc_mask, missing_mask, omega, num_steps, timestep=0):
+        """``num_steps`` steps with the buffer swap in C; returns (f_0, f_1) with the newest
+        populations in f_0, i.e. the state the reference's loop
+        ``f_0, f_1 = stepper(...); f_0, f_1 = f_1, f_0`` leaves after ``num_steps`` passes."""
+        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
+        _lib.check(
+            _lib.lib().xlbh_run(
+                self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
+                _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
+                nx, ny, nz, float(omega), int(timestep), int(num_steps), _lib.stream_of(f_0),
+            ),
+            "IncompressibleNavierStokesStepper.run",
+        )
+        _lib.mark_written(f_0, f_1)
+        if num_steps % 2 == 1:
+            return f_1, f_0
+        return f_0, f_1

@@ -1,0 +1,97 @@
+"""Store vs compute precision (mirrors xlb/precision_policy.py:8-89).
+
+The reference's members keep their names and order.  Added:
+
+* ``Precision.BF16`` and ``PrecisionPolicy.FP32BF16`` -- bfloat16 storage, FP32 arithmetic;
+* ``PrecisionPolicy.FP32FP16_SHIFTED`` / ``FP32BF16_SHIFTED`` -- populations stored as the
+  deviation ``f_l - w_l`` in FP16 / BF16 with FP32 arithmetic.  A stored field of these policies
+  holds deviations, so raw tensor values differ from ``f`` by the lattice weights; every operator
+  of this package reads and writes them in that form.
+"""
+
+from enum import Enum, auto
+
+import torch
+
+
+class Precision(Enum):
+    FP64 = auto()
+    FP32 = auto()
+    FP16 = auto()
+    UINT8 = auto()
+    BOOL = auto()
+    BF16 = auto()
+
+    @property
+    def torch_dtype(self):
+        return {
+            Precision.FP64: torch.float64,
+            Precision.FP32: torch.float32,
+            Precision.FP16: torch.float16,
+            Precision.UINT8: torch.uint8,
+            Precision.BOOL: torch.bool,
+            Precision.BF16: torch.bfloat16,
+        }[self]
+
+    @property
+    def field_id(self):
+        """Element-type id of macroscopic fields in the C ABI (XLBH_FIELD_*)."""
+        ids = {Precision.FP32: 0, Precision.FP64: 1, Precision.FP16: 2, Precision.BF16: 3}
+        if self not in ids:
+            raise ValueError(f"{self} is not a floating-point field type")
+        return ids[self]
+
+    @staticmethod
+    def from_torch(dtype):
+        for p in Precision:
+            if p.torch_dtype == dtype:
+                return p
+        raise ValueError(f"unsupported dtype {dtype}")
+
+
+class PrecisionPolicy(Enum):
+    FP64FP64 = auto()
+    FP64FP32 = auto()
+    FP64FP16 = auto()
+    FP32FP32 = auto()
+    FP32FP16 = auto()
+    FP32BF16 = auto()
+    FP32FP16_SHIFTED = auto()
+    FP32BF16_SHIFTED = auto()
+
+    @property
+    def compute_precision(self):
+        if self in (PrecisionPolicy.FP64FP64, PrecisionPolicy.FP64FP32, PrecisionPolicy.FP64FP16):
+            return Precision.FP64
+        return Precision.FP32
+
+    @property
+    def store_precision(self):
+        return {
+            PrecisionPolicy.FP64FP64: Precision.FP64,
+            PrecisionPolicy.FP64FP32: Precision.FP32,
+            PrecisionPolicy.FP64FP16: Precision.FP16,
+            PrecisionPolicy.FP32FP32: Precision.FP32,
+            PrecisionPolicy.FP32FP16: Precision.FP16,
+            PrecisionPolicy.FP32BF16: Precision.BF16,
+            PrecisionPolicy.FP32FP16_SHIFTED: Precision.FP16,
+            PrecisionPolicy.FP32BF16_SHIFTED: Precision.BF16,
+        }[self]
+
+    @property
+    def shifted(self):
+        """True when populations are stored as f_l - w_l."""
+        return self in (PrecisionPolicy.FP32FP16_SHIFTED, PrecisionPolicy.FP32BF16_SHIFTED)
+
+    @property
+    def store_format(self):
+        """Population storage format id in the C ABI (XLBH_F32 ... XLBH_BF16_SHIFTED)."""
+        base = {Precision.FP32: 0, Precision.FP64: 1, Precision.FP16: 2, Precision.BF16: 3}[self.store_precision]
+        if self.shifted:
+            base += 2
+        return base
+
+    @property
+    def compute_id(self):
+        """Compute type id in the C ABI (XLBH_COMPUTE_F32 / XLBH_COMPUTE_F64)."""
+        return 1 if self.compute_precision == Precision.FP64 else 0
