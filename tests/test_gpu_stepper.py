@@ -37,14 +37,17 @@ TOL = {
 SIXTEEN = (orc.F16, orc.BF16, orc.F16_SHIFTED, orc.BF16_SHIFTED)
 
 
-def ulp_distance(a, b):
-    """Distance in units in the last place between two arrays of 16-bit float bit patterns."""
-
-    def ordered(x):
-        x = x.astype(np.int64)
-        return np.where(x & 0x8000, -(x & 0x7FFF), x)
-
-    return np.abs(ordered(a) - ordered(b))
+def stored_ulp(raw, fmt):
+    """One unit in the last place of a 16-bit format at the largest stored magnitude of `raw`
+    (the stored deviation f - w for the shifted formats).  Rounding differences travel between
+    neighbouring cells by streaming, so a value anywhere in the field may differ by a few ulps of
+    the field's scale, including values near zero."""
+    if fmt in (orc.F16, orc.F16_SHIFTED):
+        vals, mant = raw.view(np.float16).astype(np.float64), 10
+    else:
+        vals, mant = orc.bf16_bits_to_f32(raw).astype(np.float64), 7
+    m = np.abs(vals).max()
+    return 2.0 ** (np.floor(np.log2(m)) - mant) if m > 0 else 0.0
 
 
 def compare(pair, steps, tol=None, use_run=False):
@@ -60,12 +63,15 @@ def compare(pair, steps, tol=None, use_run=False):
     dr = np.abs(ra - rb)[~solid].max()
     du = np.abs(ua - ub)[:, ~solid].max()
     if pair.fmt in SIXTEEN:
-        ulps = ulp_distance(to_np(pair.f0), pair.o_f0).max()
-        assert ulps <= 4, f"{pair.name} {pair.policy}: stored values differ by {ulps} ulps"
-    else:
-        if pair.policy == P.FP32FP32 and df > tf:
-            noise = np.abs(pair.oracle_reference_f(steps, P.FP64FP64) - fb).max()
-            tf = max(tf, 2 * noise)
+        ulp = stored_ulp(pair.o_f0, pair.fmt)
+        assert df <= 4 * ulp, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > 4 ulps ({4 * ulp:.3e})"
+    elif pair.policy == P.FP32FP32 and (df > tf or dr > tm or du > tm):
+        # FP32 noise floor of this case: the oracle's own FP32 error against an FP64 oracle run
+        ref = pair.oracle_reference_f(steps, P.FP64FP64)
+        rr, ur = moments(ref, pair.vs.c)
+        tf = max(tf, 2 * np.abs(ref - fb).max())
+        tm = max(tm, 2 * np.abs(rr - rb)[~solid].max(), 2 * np.abs(ur - ub)[:, ~solid].max())
+    if pair.fmt not in SIXTEEN:
         assert df <= tf, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > {tf:.1e}"
     assert dr <= tm and du <= tm, f"{pair.name} {pair.policy}: max|drho| = {dr:.3e}, max|du| = {du:.3e} > {tm:.1e}"
     return df, dr, du

@@ -389,3 +389,144 @@ def test_momentum_exchange_symmetric_body_has_no_lift():
     F = orc.momentum_transfer(lat, orc.F64, 1, dict(id=1, kind=orc.HALFWAY_BB), out, bc_mask, miss)
     assert F[0] > 0  # drag along the flow
     assert abs(F[1]) < 1e-12 * abs(F[0]) + 1e-15 and abs(F[2]) < 1e-12 * abs(F[0]) + 1e-15
+
+
+# ------------------------------------------------------------------ boundary-condition closed forms
+def _face_case(lat, n, face_x0=True):
+    """A face BC on x = 0 (or x = n-1) without edges; returns idx, bc_mask, missing."""
+    d, q = LATS[lat]
+    shp = grid_shape(lat, n)
+    g = np.indices(shp)
+    xs = 0 if face_x0 else n - 1
+    sel = (g[0] == xs) & (g[1] > 0) & (g[1] < n - 1)
+    if d == 3:
+        sel &= (g[2] > 0) & (g[2] < n - 1)
+    idx = np.array(np.where(sel), dtype=np.int32)
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.zeros(idx.shape[1], np.uint8), 1, bc_mask, miss)
+    return shp, sel, bc_mask, miss
+
+
+def _random_pops(lat, shp, seed):
+    rng = np.random.default_rng(seed)
+    _, w = ref_lattice(lat)
+    return w.reshape(-1, 1, 1, 1) * (1 + 0.1 * rng.standard_normal((len(w),) + shp))
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+@pytest.mark.parametrize("kind", [orc.ZOUHE_VELOCITY, orc.REGULARIZED_VELOCITY])
+def test_velocity_bc_imposes_density_and_normal_momentum(lat, kind):
+    """Zou-He velocity (bc_zouhe.py:279-312): on the x = 0 face the normal is n = -e_x, u = v e_x,
+    rho = fsum / (1 - v) with fsum = 2 sum(known) + sum(middle).  Non-equilibrium bounce-back then
+    gives sum f = rho and sum c_x f = rho v exactly; regularisation keeps both."""
+    d, q = LATS[lat]
+    c, _ = ref_lattice(lat)
+    shp, sel, bc_mask, miss = _face_case(lat, 8)
+    fpost = _random_pops(lat, shp, 11)
+    v = 0.04
+    out = orc.apply_bc(lat, orc.F64, 1, dict(id=1, kind=kind), fpost, fpost, bc_mask, miss, aux=np.full(bc_mask.size, v))
+    opp = orc.lattice(lat)["opp"]
+    mm = miss.astype(bool)
+    known = mm[opp]
+    middle = ~(mm | known)
+    fsum = 2 * (fpost * known).sum(0) + (fpost * middle).sum(0)
+    rho = fsum / (1 - v)
+    np.testing.assert_allclose(out.sum(0)[sel], rho[sel], rtol=1e-13)
+    np.testing.assert_allclose(np.tensordot(c[0], out, axes=(0, 0))[sel], rho[sel] * v, rtol=1e-12, atol=1e-15)
+    # cells off the face are untouched
+    np.testing.assert_array_equal(out[:, ~sel], fpost[:, ~sel])
+
+
+@pytest.mark.parametrize("lat", [0, 1])
+@pytest.mark.parametrize("kind", [orc.ZOUHE_PRESSURE, orc.REGULARIZED_PRESSURE])
+def test_pressure_bc_imposes_density(lat, kind):
+    """Zou-He pressure (bc_zouhe.py:314-338): rho is the prescribed value and u = (fsum/rho - 1) n;
+    the result carries exactly that density and normal momentum."""
+    d, q = LATS[lat]
+    c, _ = ref_lattice(lat)
+    shp, sel, bc_mask, miss = _face_case(lat, 8, face_x0=False)
+    fpost = _random_pops(lat, shp, 12)
+    rho_p = 1.02
+    out = orc.apply_bc(lat, orc.F64, 1, dict(id=1, kind=kind), fpost, fpost, bc_mask, miss, aux=np.full(bc_mask.size, rho_p))
+    opp = orc.lattice(lat)["opp"]
+    mm = miss.astype(bool)
+    known = mm[opp]
+    middle = ~(mm | known)
+    fsum = 2 * (fpost * known).sum(0) + (fpost * middle).sum(0)
+    un = -1 + fsum / rho_p  # along n = +e_x on the x = n-1 face
+    np.testing.assert_allclose(out.sum(0)[sel], rho_p, rtol=1e-13)
+    np.testing.assert_allclose(np.tensordot(c[0], out, axes=(0, 0))[sel], (rho_p * un)[sel], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+def test_regularized_reproduces_nonequilibrium_stress(lat):
+    """Regularisation (helper_functions_bc.py:99-122, Latt et al. 2008 eq. 45) rebuilds
+    f = feq + 9/2 w Q:Pi_neq; by fourth-order lattice isotropy the rebuilt populations carry
+    exactly the second moment Pi_neq of the Zou-He populations they replace (and rho, u)."""
+    d, q = LATS[lat]
+    c, w = ref_lattice(lat)
+    shp, sel, bc_mask, miss = _face_case(lat, 8)
+    fpost = _random_pops(lat, shp, 13)
+    v = 0.03
+    aux = np.full(bc_mask.size, v)
+    zh = orc.apply_bc(lat, orc.F64, 1, dict(id=1, kind=orc.ZOUHE_VELOCITY), fpost, fpost, bc_mask, miss, aux=aux)
+    reg = orc.apply_bc(lat, orc.F64, 1, dict(id=1, kind=orc.REGULARIZED_VELOCITY), fpost, fpost, bc_mask, miss, aux=aux)
+    rho = zh.sum(0)
+    u = np.tensordot(c, zh, axes=(1, 0)) / rho
+    feq = np.zeros_like(zh)
+    orc.equilibrium(lat, orc.F64, 1, rho, u, feq)
+    pi = lambda g: np.einsum("al,bl,l...->ab...", c, c, g)
+    np.testing.assert_allclose(pi(reg - feq)[..., sel], pi(zh - feq)[..., sel], atol=1e-14)
+    np.testing.assert_allclose(reg.sum(0)[sel], rho[sel], rtol=1e-13)
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+def test_extrapolation_outflow_keeps_uniform_flow(lat):
+    """A uniform equilibrium flow through an EquilibriumBC inlet and an ExtrapolationOutflowBC
+    outlet (bc_extrapolation_outflow.py:152-195) stays exactly uniform: every extrapolation mixes
+    equal values, so an index or sign slip (e.g. f[opp l] for f[l]) would show."""
+    d, q = LATS[lat]
+    n = 10
+    shp = grid_shape(lat, n)
+    g = np.indices(shp)
+    inlet = g[0] == 0
+    outlet = g[0] == n - 1
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    # periodic in y (and z): missing only across the x faces
+    c, _ = ref_lattice(lat)
+    xs = np.arange(n).reshape(n, 1, 1)
+    for l in range(q):
+        src = xs - c[0, l]
+        miss[l] = np.broadcast_to((src < 0) | (src >= n), shp)
+    bc_mask[0][inlet] = 1
+    bc_mask[0][outlet] = 2
+    u0 = np.array([0.03, 0.01, -0.02][:d]).reshape((d,) + (1,) * 3) * np.ones((d,) + shp)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, np.ones(shp), u0, f)
+    bcs = [dict(id=1, kind=orc.EQUILIBRIUM, rho=1.0, u=tuple(u0[:, 0, 0, 0])), dict(id=2, kind=orc.EXTRAPOLATION_OUTFLOW)]
+    # The outlet keeps, in the outgoing slot opp(l) of each missing l, the extrapolated value of
+    # direction l that the next step bounces back; for a uniform flow its fixed point is feq_l.
+    opp = orc.lattice(lat)["opp"]
+    expect = f.copy()
+    for l in range(q):
+        m = miss[l].astype(bool) & outlet
+        expect[opp[l]][m] = f[l][m]
+    out = _run(lat, 0, expect, 1.3, 12, bc_mask, miss, bcs)
+    np.testing.assert_allclose(out, expect, atol=1e-14)
+
+
+def test_do_nothing_skips_streaming():
+    """DoNothingBC (bc_do_nothing.py:52-62): the cell collides its own pre-stream populations."""
+    lat, (d, q) = 1, LATS[1]
+    shp, sel, bc_mask, miss = _face_case(lat, 6)
+    f = _random_pops(lat, shp, 14)
+    out = _run(lat, 0, f, 1.4, 1, bc_mask, miss, [dict(id=1, kind=orc.DO_NOTHING)])
+    c, _ = ref_lattice(lat)
+    rho = f.sum(0)
+    u = np.tensordot(c, f, axes=(1, 0)) / rho
+    feq = np.zeros_like(f)
+    orc.equilibrium(lat, orc.F64, 1, rho, u, feq)
+    expect = f - 1.4 * (f - feq)
+    np.testing.assert_allclose(out[:, sel], expect[:, sel], atol=1e-15)

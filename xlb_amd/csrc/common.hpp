@@ -187,10 +187,38 @@ __device__ __forceinline__ R to_raw(C v) {
     return (R)v;
 }
 
+// Cache policy of the population sweeps.  Every population is read once and written once per
+// step and the fields are far larger than L2 + MALL, so the streaming (non-temporal) forms keep
+// the sweep from evicting the lines that neighbouring lanes are still reusing.
+#ifndef XLBH_NT_LOADS
+#define XLBH_NT_LOADS 0
+#endif
+#ifndef XLBH_NT_STORES
+#define XLBH_NT_STORES 1
+#endif
+
+template <class R>
+__device__ __forceinline__ R ld_stream(const R* p) {
+#if XLBH_NT_LOADS
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
+template <class R>
+__device__ __forceinline__ void st_stream(R* p, R v) {
+#if XLBH_NT_STORES
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // Load population l (weight w) of format FMT as compute type C.
 template <int FMT, class C>
 __device__ __forceinline__ C ld_pop(const typename Fmt<FMT>::raw* p, double w) {
-  C v = raw_to<typename Fmt<FMT>::raw, C>(*p);
+  C v = raw_to<typename Fmt<FMT>::raw, C>(ld_stream(p));
   if constexpr (Fmt<FMT>::shifted) v += (C)w;
   return v;
 }
@@ -198,7 +226,7 @@ __device__ __forceinline__ C ld_pop(const typename Fmt<FMT>::raw* p, double w) {
 template <int FMT, class C>
 __device__ __forceinline__ void st_pop(typename Fmt<FMT>::raw* p, C v, double w) {
   if constexpr (Fmt<FMT>::shifted) v -= (C)w;
-  *p = to_raw<typename Fmt<FMT>::raw, C>(v);
+  st_stream(p, to_raw<typename Fmt<FMT>::raw, C>(v));
 }
 
 // Macroscopic-field element types (rho, u, Pi outputs of the standalone operators).
