@@ -215,6 +215,25 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
 int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask,
                        double* out, void* workspace, int nx, int ny, int nz, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Reverse mode of the fused step (BGK, all boundary kinds).  No counterpart function in the
+ * reference, whose gradients come from Warp's tape replaying generated adjoint kernels
+ * (examples/out_of_core/subroutine/autodiff_stepper_subroutine.py:459-466).
+ *
+ *   f_in      : the (Q, N) state the forward step read (store format `store`).
+ *   lam_out   : dL/df_out, (Q, N) in the compute type.
+ *   lam_in    : written with dL/df_in, (Q, N) in the compute type (every element is written).
+ *   grad_force: (D, N) FP64; dL/dg of the step is ADDED (FIELD and UNIFORM force modes; for the
+ *               uniform vector sum the field).  May be NULL.
+ * Other arguments as for xlbh_step.  Fields with solid cells (id 255) are not supported.
+ * ------------------------------------------------------------------------------------- */
+int xlbh_adjoint_step(int lattice, int store, int compute,
+                      const void* f_in, const void* lam_out, void* lam_in,
+                      const uint8_t* bc_mask, const uint32_t* missing_bits,
+                      const void* aux, const xlbh_bc_desc* bc_table,
+                      int force_mode, const double* force_uniform, const void* force_field,
+                      double* grad_force, int nx, int ny, int nz, double omega, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

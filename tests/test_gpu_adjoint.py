@@ -1,0 +1,218 @@
+"""GPU checks of the hand-written adjoint step (xlbh_adjoint_step) and the checkpointed rollout.
+
+The reference differentiates its Warp kernels with wp.Tape (autodiff_stepper_subroutine.py);
+its JAX-backend gradients cannot be produced here (jax is absent), so the adjoint is pinned by
+the mathematics instead:
+* central finite differences of the forward stepper in FP64 (the forward itself is pinned
+  against the oracle by the stepper tests), entry by entry on tiny grids and along random and
+  boundary-localised directions on larger ones, for every boundary kind, body force and lattice;
+* FP32 adjoints against FP64 adjoints of the same trajectory;
+* checkpoint spacing (every step, every k, sqrt(N), none) changes nothing but summation order;
+  BF16 checkpoints of f - w stay within a stated bound.
+Relative error bar: 1e-6 for FP64 against FD (the target for the method is 1e-4).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import xlb_amd as xa
+from xlb_amd.autodiff import equilibrium_torch, macroscopic_torch, rollout
+from lbm_cases import make_bcs
+
+pytestmark = pytest.mark.gpu
+
+P = xa.PrecisionPolicy
+
+ADJ_CASES = {
+    # name: (lattice, shape, boundary set, omega, force)
+    "tiny2d_channel": ("D2Q9", (8, 6), "channel2d_zouhe", 1.2, "field"),
+    "tiny2d_regularized": ("D2Q9", (8, 6), "channel2d_regularized", 1.7, "field"),
+    "tiny2d_zouhe_pressure": ("D2Q9", (8, 6), "channel2d_zouhe_pressure", 1.1, None),
+    "tiny2d_cavity": ("D2Q9", (7, 6), "cavity2d", 1.5, "uniform"),
+    "tiny3d_sphere": ("D3Q19", (8, 6, 6), "sphere3d", 1.6, "field"),
+    "tiny3d_do_nothing": ("D3Q27", (5, 4, 4), "do_nothing", 1.3, "field"),
+    "periodic2d": ("D2Q9", (24, 20), "periodic", 1.3, "field"),
+    "periodic3d_uniform": ("D3Q19", (10, 8, 6), "periodic", 1.6, "uniform"),
+    "channel2d_zouhe": ("D2Q9", (32, 16), "channel2d_zouhe", 1.2, "field"),
+    "channel2d_regularized": ("D2Q9", (32, 16), "channel2d_regularized", 1.7, "field"),
+    "cavity_fullway": ("D3Q19", (12, 12, 12), "cavity_fullway", 1.0, "field"),
+    "cavity_halfway": ("D3Q19", (12, 12, 12), "cavity_halfway", 1.6779, None),
+    "sphere3d": ("D3Q19", (24, 16, 16), "sphere3d", 1.6, "field"),
+    "sphere3d_q27": ("D3Q27", (20, 12, 12), "sphere3d", 1.6, "field"),
+    "do_nothing": ("D3Q19", (12, 10, 10), "do_nothing", 1.0, "field"),
+}
+
+TINY = [k for k in ADJ_CASES if k.startswith("tiny")]
+LARGER = [k for k in ADJ_CASES if not k.startswith("tiny")]
+
+
+class Case:
+    def __init__(self, name, policy, seed=0):
+        lat, shape, bcname, omega, force = ADJ_CASES[name]
+        vs = getattr(xa.velocity_set, lat)(policy, xa.ComputeBackend.MI355X)
+        xa.init(velocity_set=vs, default_backend=xa.ComputeBackend.MI355X, default_precision_policy=policy)
+        grid = xa.grid.grid_factory(shape, compute_backend=xa.ComputeBackend.MI355X)
+        bcs = make_bcs(bcname, grid, xa)
+        shape3 = tuple(grid.field_shape)
+        rng = np.random.default_rng(seed)
+        cdt = policy.compute_precision.torch_dtype
+        self.ff = None
+        fv = None
+        if force == "field":
+            self.ff = torch.from_numpy(2e-4 * rng.standard_normal((vs.d,) + shape3)).to(cdt).cuda()
+        elif force == "uniform":
+            fv = np.array([1e-4, -2e-4, 5e-5][: vs.d])
+        self.st = xa.operator.stepper.IncompressibleNavierStokesStepper(
+            grid, bcs, collision_type="BGK", force_vector=fv, force_field=self.ff
+        )
+        _, _, self.bc_mask, self.missing = self.st.prepare_fields()
+        # generic state: equilibrium of a random flow plus a non-equilibrium part
+        rho = torch.from_numpy(1.0 + 0.01 * rng.standard_normal((1,) + shape3)).cuda()
+        u = torch.from_numpy(0.03 * rng.standard_normal((vs.d,) + shape3)).cuda()
+        w = torch.tensor(np.asarray(vs.w, dtype=np.float64), device="cuda").reshape((-1,) + (1,) * 3)
+        f = equilibrium_torch(rho, u, vs) + 2e-3 * w * torch.from_numpy(rng.standard_normal((vs.q,) + shape3)).cuda()
+        self.f0 = f.to(self.st.store_dtype).contiguous()
+        self.W = torch.from_numpy(rng.standard_normal((vs.q,) + shape3)).cuda()
+        self.vs, self.omega, self.policy = vs, omega, policy
+
+    def forward(self, f0, steps, ff=None):
+        if ff is not None:
+            self.st.collision.force_field = ff
+        a = f0.clone()
+        b = torch.empty_like(a)
+        for t in range(steps):
+            self.st(a, b, self.bc_mask, self.missing, self.omega, t)
+            a, b = b, a
+        return a
+
+    def loss(self, fT):
+        fT = fT.double()
+        _, u = macroscopic_torch(fT, self.vs)
+        return (self.W * fT).sum() + 50.0 * (u * u).sum()
+
+    def gradients(self, steps, every=None, ckpt="store"):
+        f0 = self.f0.clone().requires_grad_(True)
+        ff = self.ff.clone().requires_grad_(True) if self.ff is not None else None
+        fT = rollout(self.st, f0, self.bc_mask, self.missing, self.omega, steps, force_field=ff,
+                     checkpoint_every=every, checkpoint_dtype=ckpt)
+        self.loss(fT).backward()
+        return f0.grad.double(), (ff.grad.double() if ff is not None else None)
+
+    def loss_at(self, f0, steps, ff=None):
+        return float(self.loss(self.forward(f0, steps, ff)))
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_adjoint_elementwise_fd(name):
+    """Every entry of dL/df0 and dL/dg against central FD (FP64, 3 steps, checkpoint every 2)."""
+    c = Case(name, P.FP64FP64)
+    steps, eps = 3, 1e-6
+    g_f, g_g = c.gradients(steps, every=2)
+    fd = torch.zeros_like(g_f)
+    flat0 = c.f0.reshape(-1)
+    for i in range(flat0.numel()):
+        fp, fm = c.f0.clone(), c.f0.clone()
+        fp.view(-1)[i] += eps
+        fm.view(-1)[i] -= eps
+        fd.view(-1)[i] = (c.loss_at(fp, steps) - c.loss_at(fm, steps)) / (2 * eps)
+    scale = fd.abs().max().item()
+    err = (g_f - fd).abs().max().item()
+    assert err <= 1e-6 * scale, f"{name}: max|adj - fd| = {err:.3e} (scale {scale:.3e})"
+    if c.ff is not None:
+        fdg = torch.zeros_like(g_g)
+        for i in range(c.ff.numel()):
+            gp, gm = c.ff.clone(), c.ff.clone()
+            gp.view(-1)[i] += eps
+            gm.view(-1)[i] -= eps
+            fdg.view(-1)[i] = (c.loss_at(c.f0, steps, gp) - c.loss_at(c.f0, steps, gm)) / (2 * eps)
+        c.st.collision.force_field = c.ff
+        scale = fdg.abs().max().item()
+        err = (g_g - fdg).abs().max().item()
+        assert err <= 1e-6 * scale, f"{name}: force grad max|adj - fd| = {err:.3e} (scale {scale:.3e})"
+
+
+def _near_boundary(c):
+    """1 at boundary cells and their face neighbours, else 0 (shape of bc_mask); None if no BC."""
+    bc = (c.bc_mask != 0).double()
+    if not bc.any():
+        return None
+    near = bc.clone()
+    for ax in range(1, bc.dim()):
+        if bc.shape[ax] > 1:
+            near = torch.maximum(near, torch.roll(bc, 1, ax))
+            near = torch.maximum(near, torch.roll(bc, -1, ax))
+    return near
+
+
+def _directions(shape, near, seed):
+    """A global random direction and the same one confined to the cells of ``near``."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    v = torch.randn(shape, generator=g, dtype=torch.float64).cuda()
+    dirs = [("global", v)]
+    if near is not None:
+        dirs.append(("boundary", v * near))
+    return dirs
+
+
+@pytest.mark.parametrize("name", LARGER)
+def test_adjoint_directional_fd(name):
+    """Directional derivatives along random and boundary-localised directions (FP64, 8 steps,
+    sqrt-N checkpointing)."""
+    c = Case(name, P.FP64FP64, seed=1)
+    steps, eps = 8, 1e-6
+    g_f, g_g = c.gradients(steps)
+    near = _near_boundary(c)
+    for tag, v in _directions(c.f0.shape, near, 3):
+        fd = (c.loss_at(c.f0 + eps * v, steps) - c.loss_at(c.f0 - eps * v, steps)) / (2 * eps)
+        ad = float((g_f * v).sum())
+        assert rel(ad, fd) <= 1e-6, f"{name} f0/{tag}: adjoint {ad:.10e} vs FD {fd:.10e}"
+    if c.ff is not None:
+        for tag, v in _directions(c.ff.shape, near, 5):
+            fd = (c.loss_at(c.f0, steps, c.ff + eps * v) - c.loss_at(c.f0, steps, c.ff - eps * v)) / (2 * eps)
+            c.st.collision.force_field = c.ff
+            ad = float((g_g * v).sum())
+            assert rel(ad, fd) <= 1e-6, f"{name} force/{tag}: adjoint {ad:.10e} vs FD {fd:.10e}"
+
+
+@pytest.mark.parametrize("name", ["channel2d_zouhe", "sphere3d", "periodic3d_uniform"])
+def test_adjoint_fp32_matches_fp64(name):
+    """FP32 compute/store adjoint of the same case against the FP64 one: relative L2 <= 1e-5."""
+    g64, gg64 = Case(name, P.FP64FP64, seed=2).gradients(10)
+    g32, gg32 = Case(name, P.FP32FP32, seed=2).gradients(10)
+    r = ((g32 - g64).norm() / g64.norm()).item()
+    assert r <= 1e-5, f"{name}: FP32 vs FP64 adjoint rel L2 {r:.3e}"
+    if gg64 is not None:
+        r = ((gg32 - gg64).norm() / gg64.norm()).item()
+        assert r <= 1e-4, f"{name}: FP32 vs FP64 force gradient rel L2 {r:.3e}"
+
+
+def test_checkpoint_spacing_invariance():
+    """Gradients do not depend on the checkpoint spacing beyond summation order (atomic adds at
+    boundary cells); BF16 checkpoints of f - w stay within 1e-4 relative L2."""
+    steps = 13
+    c = Case("channel2d_zouhe", P.FP64FP64, seed=4)
+    ref_f, ref_g = c.gradients(steps, every=steps)
+    for every in (1, 3, None):
+        g_f, g_g = c.gradients(steps, every=every)
+        assert ((g_f - ref_f).norm() / ref_f.norm()).item() <= 1e-12
+        assert ((g_g - ref_g).norm() / ref_g.norm()).item() <= 1e-12
+    c32 = Case("channel2d_zouhe", P.FP32FP32, seed=4)
+    e_f, e_g = c32.gradients(steps, every=4)
+    b_f, b_g = c32.gradients(steps, every=4, ckpt="bf16")
+    r = ((b_f - e_f).norm() / e_f.norm()).item()
+    assert r <= 1e-4, f"BF16 checkpoints: rel L2 {r:.3e}"
+
+
+def test_adjoint_rejects_bad_arguments():
+    c = Case("tiny2d_cavity", P.FP64FP64)
+    lam = torch.zeros_like(c.f0)
+    with pytest.raises(ValueError):
+        c.st.adjoint_step(c.f0, lam, lam, c.bc_mask, c.missing, c.omega)
+    kbc = xa.operator.stepper.IncompressibleNavierStokesStepper(c.st.grid, [], collision_type="KBC")
+    with pytest.raises(NotImplementedError):
+        kbc.adjoint_step(c.f0, lam, torch.zeros_like(lam), c.bc_mask, c.missing, c.omega)

@@ -24,3 +24,4 @@ import xlb_amd.operator.boundary_masker
 import xlb_amd.operator.force
 import xlb_amd.operator.stepper
 import xlb_amd.helper
+import xlb_amd.autodiff

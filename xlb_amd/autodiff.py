@@ -1,0 +1,129 @@
+"""Differentiable rollouts of the fused step (reverse mode by the hand-written adjoint kernel).
+
+``rollout(stepper, f0, bc_mask, missing_mask, omega, num_steps, force_field=None, ...)`` returns the
+populations after ``num_steps`` steps as a tensor that participates in torch autograd: gradients
+flow to ``f0`` and to the per-cell body-force field.  Memory is bounded by checkpointing: every
+``checkpoint_every`` steps (default ceil(sqrt(num_steps))) the state is kept, optionally in a
+16-bit shifted form (f - w stored as bfloat16); the backward pass recomputes each segment from
+its checkpoint and runs the adjoint step (xlbh_adjoint_step) through it in reverse.
+
+The reference differentiates with Warp's tape over k-step segments with stored checkpoints
+(examples/out_of_core/autodiff_lbm.py:64-127, subroutine/autodiff_stepper_subroutine.py:287-555);
+here the adjoint is a hand-written kernel and the per-cell force gradient is accumulated in FP64
+over cells and time.
+
+Also provides differentiable torch forms of the equilibrium and of the macroscopic moments for
+building initial states and losses.
+"""
+
+import math
+
+import numpy as np
+import torch
+
+from xlb_amd.operator.collision import ForcedCollision
+
+
+def equilibrium_torch(rho, u, velocity_set):
+    """feq(rho, u) as differentiable torch ops (quadratic_equilibrium.py:18-25); rho (1,...),
+    u (d,...) -> (q,...)."""
+    c = torch.as_tensor(np.asarray(velocity_set.c, dtype=np.float64), dtype=u.dtype, device=u.device)
+    w = torch.as_tensor(np.asarray(velocity_set.w, dtype=np.float64), dtype=u.dtype, device=u.device)
+    shape = (-1,) + (1,) * (u.dim() - 1)
+    cu = 3.0 * torch.tensordot(c.T, u, dims=([1], [0]))
+    usqr = 1.5 * (u * u).sum(0, keepdim=True)
+    return rho * w.reshape(shape) * (1.0 + cu * (1.0 + 0.5 * cu) - usqr)
+
+
+def macroscopic_torch(f, velocity_set):
+    """rho, u of populations f (q, ...) as differentiable torch ops."""
+    c = torch.as_tensor(np.asarray(velocity_set.c, dtype=np.float64), dtype=f.dtype, device=f.device)
+    rho = f.sum(0, keepdim=True)
+    u = torch.tensordot(c, f, dims=([1], [0])) / rho
+    return rho, u
+
+
+class _Checkpoint:
+    def __init__(self, f, mode, w):
+        self.mode = mode
+        if mode == "bf16":
+            self.data = (f.float() - w).to(torch.bfloat16)
+        else:
+            self.data = f.clone()
+
+    def restore(self, out, w):
+        if self.mode == "bf16":
+            out.copy_(self.data.float() + w)
+        else:
+            out.copy_(self.data)
+        return out
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f0, force_field, stepper, bc_mask, missing_mask, omega, num_steps, every, ckpt_mode):
+        col = stepper.collision
+        if force_field is not None:
+            if not isinstance(col, ForcedCollision) or col.force_field is None:
+                raise ValueError("a differentiable force field needs a stepper built with force_field=...")
+            col.force_field = force_field.detach().to(stepper.compute_dtype).contiguous()
+        w = torch.as_tensor(np.asarray(stepper.velocity_set.w), dtype=torch.float32, device=f0.device).reshape(-1, 1, 1, 1)
+        a = f0.detach().clone().contiguous()
+        b = torch.empty_like(a)
+        ckpts = {}
+        for t in range(num_steps):
+            if t % every == 0:
+                ckpts[t] = _Checkpoint(a, ckpt_mode, w)
+            stepper(a, b, bc_mask, missing_mask, omega, t)
+            a, b = b, a
+        ctx.stepper = stepper
+        ctx.masks = (bc_mask, missing_mask)
+        ctx.omega = omega
+        ctx.num_steps = num_steps
+        ctx.every = every
+        ctx.ckpts = ckpts
+        ctx.w = w
+        ctx.has_force = force_field is not None
+        ctx.force_shape = None if force_field is None else tuple(force_field.shape)
+        ctx.force_dtype = None if force_field is None else force_field.dtype
+        return a
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        st = ctx.stepper
+        bc_mask, missing_mask = ctx.masks
+        cdt = st.compute_dtype
+        lam = grad_out.detach().to(cdt).contiguous().clone()
+        lam_next = torch.empty_like(lam)
+        grad_force = None
+        if ctx.has_force:
+            grad_force = torch.zeros(ctx.force_shape, dtype=torch.float64, device=lam.device)
+        n, k = ctx.num_steps, ctx.every
+        starts = sorted(ctx.ckpts.keys())
+        seg_buf = [torch.empty(lam.shape, dtype=st.store_dtype, device=lam.device) for _ in range(min(k, n))]
+        for s in reversed(starts):
+            length = min(k, n - s)
+            # recompute f_s .. f_{s+length-1}
+            ctx.ckpts[s].restore(seg_buf[0], ctx.w)
+            for j in range(1, length):
+                st(seg_buf[j - 1], seg_buf[j], bc_mask, missing_mask, ctx.omega, s + j - 1)
+            for j in reversed(range(length)):
+                st.adjoint_step(seg_buf[j], lam, lam_next, bc_mask, missing_mask, ctx.omega, grad_force)
+                lam, lam_next = lam_next, lam
+        g_f0 = lam.to(grad_out.dtype)
+        g_force = grad_force.to(ctx.force_dtype) if grad_force is not None else None
+        return g_f0, g_force, None, None, None, None, None, None, None
+
+
+def rollout(stepper, f0, bc_mask, missing_mask, omega, num_steps, force_field=None, checkpoint_every=None,
+            checkpoint_dtype="store"):
+    """Differentiable ``num_steps``-step rollout; see the module docstring.
+
+    checkpoint_dtype: "store" keeps checkpoints in the stepper's store dtype, "bf16" keeps f - w as
+    bfloat16 (half the memory; the recomputed trajectory then starts from the rounded state)."""
+    if num_steps < 1:
+        raise ValueError("num_steps must be >= 1")
+    every = checkpoint_every or max(1, int(math.ceil(math.sqrt(num_steps))))
+    if checkpoint_dtype not in ("store", "bf16"):
+        raise ValueError("checkpoint_dtype must be 'store' or 'bf16'")
+    return _Rollout.apply(f0, force_field, stepper, bc_mask, missing_mask, omega, int(num_steps), int(every), checkpoint_dtype)

@@ -1,0 +1,406 @@
+// Reverse mode of the fused step (BGK): given the state f^t that a forward step read and
+// lam_out = dL/df^{t+1}, produce lam_in = dL/df^t and accumulate dL/dg for a per-cell body force.
+//
+// Forward per cell x (step.hpp, nse_stepper.py:344-383):  f_ps(x) = pull(f^t)  ->  h = BC_stream(f_ps)
+//   -> out = BGK(h) (+ exact-difference force)  ->  BC_collision / extrapolation update  ->  f^{t+1}(x).
+// Reverse per cell: lam_eff = lam_out(x) with the slots the BC overwrote after collision zeroed;
+// G_h = (d out / d h)^T lam_eff (closed form below); then G_ps = (d h / d f_ps)^T G_h.
+// The transpose of the pull is a push: f_ps_k(x) = f^t_k(x - c_k), so lam_in_k(x - c_k) = G_ps_k(x).
+// Every slot of lam_in has exactly one pushing source, so kernel A writes lam_in as a permutation
+// without atomics.  Contributions that do not follow the pull -- reads of the cell's own pre-stream
+// populations (bounce-back, do-nothing, the extrapolation stash), extrapolation neighbour reads and
+// the Zou-He / regularized Jacobian -- exist only at boundary cells and are added by kernel B.
+//
+// BGK with exact-difference force g: out_l = (1-w) h_l + a feq_l(rho, u) + [g] feq_l(rho, u+g),
+// a = w - 1 with a force, w without.  With feq_l(rho, v) = rho w_l P_l(v),
+// P_l(v) = 1 + 3 c_l.v + 9/2 (c_l.v)^2 - 3/2 v.v, d rho/d h_k = 1, d v/d h_k = (c_k - u)/rho:
+//   sum_l lam_l d feq_l(rho, v)/d h_k = A(v) + sum_d B_d(v) (c_dk - u_d),
+//   A(v) = sum_l lam_l w_l P_l(v),  B_d(v) = sum_l lam_l w_l (3 c_dl + 9 (c_l.v) c_dl - 3 v_d),
+//   G_h_k = (1-w) lam_k + a [A(u) + B(u).(c_k - u)] + [g] [A(u+g) + B(u+g).(c_k - u)],
+//   dL/dg_d = rho B_d(u+g).
+#include "step.hpp"
+
+namespace xlbh {
+
+// ------------------------------------------------------------------------------------------
+// Forward-mode dual numbers (value, derivative), used to differentiate the Zou-He / regularized
+// functional at the few cells that carry it.
+// ------------------------------------------------------------------------------------------
+template <class C>
+struct Dual {
+  C v, d;
+  __device__ Dual() : v(0), d(0) {}
+  template <class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type>
+  __device__ Dual(S a) : v((C)a), d(0) {}
+  __device__ Dual(C a, C b) : v(a), d(b) {}
+  __device__ explicit operator C() const { return v; }
+};
+template <class C>
+__device__ __forceinline__ Dual<C> operator+(Dual<C> a, Dual<C> b) { return {a.v + b.v, a.d + b.d}; }
+template <class C>
+__device__ __forceinline__ Dual<C> operator-(Dual<C> a, Dual<C> b) { return {a.v - b.v, a.d - b.d}; }
+template <class C>
+__device__ __forceinline__ Dual<C> operator-(Dual<C> a) { return {-a.v, -a.d}; }
+template <class C>
+__device__ __forceinline__ Dual<C> operator*(Dual<C> a, Dual<C> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <class C>
+__device__ __forceinline__ Dual<C> operator/(Dual<C> a, Dual<C> b) {
+  const C inv = C(1) / b.v;
+  return {a.v * inv, (a.d - a.v * inv * b.d) * inv};
+}
+template <class C>
+__device__ __forceinline__ Dual<C>& operator+=(Dual<C>& a, Dual<C> b) { return a = a + b; }
+template <class C>
+__device__ __forceinline__ Dual<C>& operator-=(Dual<C>& a, Dual<C> b) { return a = a - b; }
+template <class C>
+__device__ __forceinline__ Dual<C>& operator*=(Dual<C>& a, Dual<C> b) { return a = a * b; }
+
+// ------------------------------------------------------------------------------------------
+// Collision adjoint (BGK + optional force), in place: lam (Q, the post-collision adjoint) ->
+// G_h (Q).  Also returns dL/dg in gg (D values).
+// ------------------------------------------------------------------------------------------
+template <int LAT, class C>
+__device__ __forceinline__ void eq_moments(const C* lam, const C* u, const C* v, C& A, C* B) {
+  constexpr LatTables T = make_tables(LAT);
+  A = C(0);
+  B[0] = B[1] = B[2] = C(0);
+  C vv = C(0);
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) vv += v[d] * v[d];
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    C cv = C(0);
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) cv += C(T.c[d][l]) * v[d];
+    const C lw = lam[l] * C(T.w[l]);
+    A += lw * (C(1) + C(3) * cv + C(4.5) * cv * cv - C(1.5) * vv);
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) B[d] += lw * (C(3) * C(T.c[d][l]) + C(9) * cv * C(T.c[d][l]) - C(3) * v[d]);
+  }
+  (void)u;
+}
+
+template <int LAT, class C>
+__device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, bool has_force, const C* F, C* gg) {
+  constexpr LatTables T = make_tables(LAT);
+  C rho, u[3];
+  macroscopic<LAT>(h, rho, u);
+  const C a = has_force ? omega - C(1) : omega;
+  C A0, B0[3];
+  eq_moments<LAT>(lam, u, u, A0, B0);
+  C A1 = C(0), B1[3] = {C(0), C(0), C(0)};
+  if (has_force) {
+    C ug[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ug[d] = u[d] + F[d];
+    eq_moments<LAT>(lam, u, ug, A1, B1);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gg[d] = has_force ? rho * B1[d] : C(0);
+#pragma unroll
+  for (int k = 0; k < T.q; ++k) {
+    C s = a * A0 + A1;
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) s += (a * B0[d] + B1[d]) * (C(T.c[d][k]) - u[d]);
+    lam[k] = (C(1) - omega) * lam[k] + s;
+  }
+}
+
+struct AdjParams {
+  const void* f_in;
+  const void* lam_out;
+  void* lam_in;
+  const uint8_t* bc_mask;
+  const uint32_t* missing;
+  const void* aux;
+  const xlbh_bc_desc* table;
+  const void* force_field;
+  double* grad_force;
+  double fu[3];
+  int force_mode;
+  double omega;
+  Sweep sw;
+  uint32_t N;
+};
+
+// The streaming-step view of one cell: post-BC populations h, the adjoint of the output with the
+// slots overwritten after collision removed, and the BC kind.  Shared by kernels A and B.
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, int x, int y, int z, int kind,
+                                            uint32_t mm, const xlbh_bc_desc* d, C* fps, C* h, C* lam) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  using R = typename Fmt<FMT>::raw;
+  const R* fin = static_cast<const R*>(p.f_in);
+  const C* lo = static_cast<const C*>(p.lam_out);
+  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
+    fps[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
+    lam[l] = lo[(size_t)l * p.N + cell];
+    h[l] = fps[l];
+  }
+  if (kind == XLBH_BC_EQUILIBRIUM) {
+    const C rb = (C)d->rho;
+    const C ub[3] = {(C)d->u[0], (C)d->u[1], (C)d->u[2]};
+    const C us = usqr15<LAT>(ub);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) h[l] = feq_l<LAT>(l, rb, ub, us);
+  } else if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+      if (bit(mm, l)) h[l] = ld_pop<FMT, C>(fin + (size_t)T.opp[l] * p.N + cell, T.w[T.opp[l]]);
+  } else if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) h[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + cell, T.w[l]);
+  } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
+    const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
+    zouhe_cell<LAT>(h, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
+                    kind >= XLBH_BC_REGULARIZED_VELOCITY);
+  }
+  if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+      if (bit(mm, l)) lam[T.opp[l]] = C(0);  // collision output of slot opp(l) is replaced by the stash
+  }
+}
+
+template <int LAT, class C>
+__device__ __forceinline__ void cell_force(const AdjParams& p, uint32_t cell, C* F) {
+  constexpr LatTables T = make_tables(LAT);
+  F[0] = F[1] = F[2] = C(0);
+  if (p.force_mode == XLBH_FORCE_UNIFORM) {
+    F[0] = (C)p.fu[0];
+    F[1] = (C)p.fu[1];
+    F[2] = (C)p.fu[2];
+  } else if (p.force_mode == XLBH_FORCE_FIELD) {
+    const C* ff = static_cast<const C*>(p.force_field);
+#pragma unroll
+    for (int dd = 0; dd < T.d; ++dd) F[dd] = ff[(size_t)dd * p.N + cell];
+  }
+}
+
+__device__ __forceinline__ bool heavy_kind(int kind) {
+  return kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE;
+}
+
+// Kernel A: every cell pushes G_ps into lam_in (a permutation) and adds its force gradient.
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+  int id = p.bc_mask ? p.bc_mask[cell] : 0;
+  int kind = XLBH_BC_NONE;
+  uint32_t mm = 0;
+  const xlbh_bc_desc* d = nullptr;
+  if (id != 0) {
+    d = p.table + id;
+    kind = d->kind;
+    mm = p.missing[cell];
+  }
+  C fps[Q], h[Q], lam[Q];
+  adj_prepare<LAT, FMT, C>(p, cell, x, y, z, kind, mm, d, fps, h, lam);
+  (void)fps;
+  C G[Q];
+  if (id == 255) {
+    // solid cells are not updated by the forward step: nothing flows back through their pulls
+#pragma unroll
+    for (int k = 0; k < Q; ++k) G[k] = C(0);
+  } else if (kind == XLBH_BC_FULLWAY_BB) {
+#pragma unroll
+    for (int k = 0; k < Q; ++k) G[k] = lam[T.opp[k]];
+  } else if (heavy_kind(kind) || kind == XLBH_BC_EQUILIBRIUM || kind == XLBH_BC_DO_NOTHING) {
+    // no dependence on the pulled values through this cell's pull (Zou-He is added by kernel B)
+#pragma unroll
+    for (int k = 0; k < Q; ++k) G[k] = C(0);
+    if (kind == XLBH_BC_EQUILIBRIUM && p.force_mode != XLBH_FORCE_NONE && p.grad_force) {
+      C F[3], gg[3];
+      cell_force<LAT>(p, cell, F);
+      collide_adjoint<LAT>(h, lam, (C)p.omega, true, F, gg);
+#pragma unroll
+      for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
+    }
+  } else {
+    C F[3], gg[3];
+    cell_force<LAT>(p, cell, F);
+    const bool has_force = p.force_mode != XLBH_FORCE_NONE;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) G[k] = lam[k];
+    collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg);
+    if (has_force && p.grad_force) {
+#pragma unroll
+      for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
+    }
+    if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+      for (int k = 0; k < Q; ++k)
+        if (bit(mm, k)) G[k] = C(0);  // replaced by the own-cell value (kernel B)
+    }
+  }
+  C* li = static_cast<C*>(p.lam_in);
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+    li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = G[k];
+  }
+}
+
+// Kernel B: boundary cells add the contributions that do not follow the pull.
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+  const int id = p.bc_mask[cell];
+  if (id == 0 || id == 255) return;
+  const xlbh_bc_desc* d = p.table + id;
+  const int kind = d->kind;
+  if (kind != XLBH_BC_HALFWAY_BB && kind != XLBH_BC_EXTRAPOLATION_OUTFLOW && kind != XLBH_BC_DO_NOTHING &&
+      !heavy_kind(kind))
+    return;
+  const uint32_t mm = p.missing[cell];
+  C fps[Q], h[Q], lam[Q];
+  adj_prepare<LAT, FMT, C>(p, cell, x, y, z, kind, mm, d, fps, h, lam);
+  C* li = static_cast<C*>(p.lam_in);
+  auto add = [&](int k, uint32_t c, C v) { atomicAdd(li + (size_t)k * p.N + c, v); };
+  C F[3], gg[3];
+  cell_force<LAT>(p, cell, F);
+  const bool has_force = p.force_mode != XLBH_FORCE_NONE;
+  C G[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) G[k] = lam[k];
+  collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg);
+
+  if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+      if (bit(mm, l)) add(T.opp[l], cell, G[l]);  // h_l = f_own[opp l]
+  }
+  if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+    // stash of slot opp(l): (1 - cs) f_own[opp l] + cs f^t_l(x - c_l - n)
+    int n[3];
+    bc_normal<LAT>(mm, n);
+    const C cs = C(0.57735026918962584);
+    const C* lo = static_cast<const C*>(p.lam_out);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      if (bit(mm, l)) {
+        const C ls = lo[(size_t)T.opp[l] * p.N + cell];
+        add(T.opp[l], cell, (C(1) - cs) * ls);
+        const int sx = wrap(x - (T.c[0][l] + n[0]), nx);
+        const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
+        const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
+        add(l, ((uint32_t)sx * ny + sy) * nz + sz, cs * ls);
+      }
+    }
+  }
+  if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+    for (int k = 0; k < Q; ++k) add(k, cell, G[k]);
+  }
+  if (heavy_kind(kind) || kind == XLBH_BC_DO_NOTHING) {
+    if (has_force && p.grad_force) {
+#pragma unroll
+      for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
+    }
+  }
+  if (heavy_kind(kind)) {
+    // G_ps = (d h / d f_ps)^T G_h: one dual pass per input direction k.
+    const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
+    const bool vel = (kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY);
+    const bool reg = kind >= XLBH_BC_REGULARIZED_VELOCITY;
+    for (int k = 0; k < Q; ++k) {
+      if (bit(mm, k)) continue;  // overwritten by the BC: no dependence
+      Dual<C> hd[Q];
+#pragma unroll
+      for (int l = 0; l < Q; ++l) hd[l] = Dual<C>(fps[l], l == k ? C(1) : C(0));
+      zouhe_cell<LAT>(hd, mm, Dual<C>(pv), vel, reg);
+      C s = C(0);
+#pragma unroll
+      for (int l = 0; l < Q; ++l) s += G[l] * hd[l].d;
+      const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+      add(k, ((uint32_t)sx * ny + sy) * nz + sz, s);
+    }
+  }
+}
+
+template <int LAT, int FMT, class C>
+int launch_adjoint(const AdjParams& p, hipStream_t s) {
+  hipLaunchKernelGGL((k_adjoint_push<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
+  XLBH_LAUNCH_CHECK();
+  if (p.bc_mask) {
+    hipLaunchKernelGGL((k_adjoint_bc<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
+    XLBH_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace xlbh
+
+using namespace xlbh;
+
+extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void* f_in, const void* lam_out,
+                                 void* lam_in, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                                 const void* aux, const xlbh_bc_desc* bc_table, int force_mode,
+                                 const double* force_uniform, const void* force_field, double* grad_force, int nx,
+                                 int ny, int nz, double omega, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
+  XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
+  XLBH_REQUIRE(!bc_mask || (missing_bits && bc_table), "bc_mask given without missing bits or BC table");
+  XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
+  XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
+  XLBH_REQUIRE(force_mode != XLBH_FORCE_FIELD || force_field, "force field pointer is null");
+  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 31), "adjoint_step supports fewer than 2^31 cells");
+  AdjParams p;
+  p.f_in = f_in;
+  p.lam_out = lam_out;
+  p.lam_in = lam_in;
+  p.bc_mask = bc_mask;
+  p.missing = missing_bits;
+  p.aux = aux;
+  p.table = bc_table;
+  p.force_field = force_field;
+  p.grad_force = grad_force;
+  p.force_mode = force_mode;
+  for (int d = 0; d < 3; ++d) p.fu[d] = force_mode == XLBH_FORCE_UNIFORM ? force_uniform[d] : 0.0;
+  p.omega = omega;
+  p.sw = make_sweep(nx, ny, nz);
+  p.N = (uint32_t)((int64_t)nx * ny * nz);
+  hipStream_t s = (hipStream_t)stream;
+  auto go = [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    if (compute == XLBH_COMPUTE_F32) {
+      switch (store) {
+        case XLBH_F32: return launch_adjoint<LAT, XLBH_F32, float>(p, s);
+        case XLBH_F16: return launch_adjoint<LAT, XLBH_F16, float>(p, s);
+        case XLBH_BF16: return launch_adjoint<LAT, XLBH_BF16, float>(p, s);
+        case XLBH_F16_SHIFTED: return launch_adjoint<LAT, XLBH_F16_SHIFTED, float>(p, s);
+        case XLBH_BF16_SHIFTED: return launch_adjoint<LAT, XLBH_BF16_SHIFTED, float>(p, s);
+      }
+    } else if (compute == XLBH_COMPUTE_F64) {
+      switch (store) {
+        case XLBH_F64: return launch_adjoint<LAT, XLBH_F64, double>(p, s);
+        case XLBH_F32: return launch_adjoint<LAT, XLBH_F32, double>(p, s);
+      }
+    }
+    set_error("unsupported (store, compute) combination for the adjoint step");
+    return 1;
+  };
+  switch (lattice) {
+    case XLBH_D2Q9: return go(std::integral_constant<int, XLBH_D2Q9>{});
+    case XLBH_D3Q19: return go(std::integral_constant<int, XLBH_D3Q19>{});
+    case XLBH_D3Q27: return go(std::integral_constant<int, XLBH_D3Q27>{});
+  }
+  set_error("unknown lattice id");
+  return 1;
+}

@@ -50,15 +50,6 @@ struct Planes {
   __device__ __forceinline__ R* at_w(int l, O cell) const { return const_cast<R*>(at(l, cell)); }
 };
 
-// Occupancy request of the step kernel (waves per SIMD; 0 = compiler's choice).
-#ifndef XLBH_STEP_WAVES
-#define XLBH_STEP_WAVES 0
-#endif
-#if XLBH_STEP_WAVES > 0
-#define XLBH_STEP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(XLBH_STEP_WAVES, XLBH_STEP_WAVES)))
-#else
-#define XLBH_STEP_OCCUPANCY
-#endif
 
 // Pull-source offsets x - c for c in {-1, 0, +1} per axis, with periodic wrap (stream.py:64-81).
 template <class O>
@@ -168,7 +159,7 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
 
 // One thread per cell: Q pulls, the update, Q stores.
 template <int LAT, int COLL, int FMT, class C, bool WIDE>
-__global__ __launch_bounds__(256) XLBH_STEP_OCCUPANCY void k_step(StepParams p) {
+__device__ __forceinline__ void step_body(const StepParams& p) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   using R = typename Fmt<FMT>::raw;
@@ -198,6 +189,18 @@ __global__ __launch_bounds__(256) XLBH_STEP_OCCUPANCY void k_step(StepParams p) 
 #pragma unroll
   for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout.at_w(l, cell), f[l], T.w[l]);
 }
+
+// Occupancy.  Measured on the 256^3 D3Q19 sweep: 4-byte populations stream best at 4 waves per
+// SIMD (more waves add concurrent streams without adding bandwidth); 2-byte populations want as
+// many waves as the registers allow.  The 4- and 8-byte kernels are launched with an LDS
+// reservation (kStepLdsCap, otherwise unused) that admits 4 blocks = 4 waves per SIMD per CU.
+constexpr unsigned kStepLdsCap = 36 * 1024;
+
+template <int LAT, int COLL, int FMT, class C, bool WIDE>
+__global__ __launch_bounds__(256) void k_step(StepParams p) {
+  step_body<LAT, COLL, FMT, C, WIDE>(p);
+}
+
 
 // ------------------------------------------------------------------------------------------
 // Direction-split step (BGK).  A block of W waves updates 64 consecutive cells; wave v pulls
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(64 * W) XLBH_SPLIT_OCCUPANCY void k_step_split(Step
 
 // Waves per block of the direction-split step (0 = use the one-thread-per-cell kernel).
 #ifndef XLBH_SPLIT_W
-#define XLBH_SPLIT_W 4
+#define XLBH_SPLIT_W 0
 #endif
 
 // Host launcher, instantiated per lattice in step_<lattice>.hip.
@@ -377,10 +380,11 @@ int launch_step_t(const StepCall& c) {
     else
       hipLaunchKernelGGL((k_step_split<LAT, FMT, C, false, W>), grid, dim3(64 * W), 0, c.stream, p);
   } else {
+    const unsigned lds = sizeof(typename Fmt<FMT>::raw) >= 4 ? kStepLdsCap : 0u;
     if (wide)
-      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true>), sweep_grid(p.sw), dim3(256), 0, c.stream, p);
+      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
     else
-      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), 0, c.stream, p);
+      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
   }
   XLBH_LAUNCH_CHECK();
   return 0;

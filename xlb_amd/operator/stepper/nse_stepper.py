@@ -82,6 +82,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         self._table = None
         self._aux = None
         self._has_bc_cells = True
+        self._has_solid = False
 
     # ------------------------------------------------------------------ fields
     def prepare_fields(self, initializer=None):
@@ -151,6 +152,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         self._bits = packed_missing(missing_mask, self.lattice_id)
         self._table = self._build_table(bc_mask.device)
         self._has_bc_cells = bool(bc_mask.any().item())
+        self._has_solid = self._has_bc_cells and bool((bc_mask == 255).any().item())
         nx, ny, nz = field_extents(bc_mask)
         aux_bcs = [bc for bc in self.boundary_conditions if bc.needs_aux_init]
         if aux_bcs and (self._aux is None or self._aux.numel() != nx * ny * nz or self._state_key is not None):
@@ -179,9 +181,10 @@ class IncompressibleNavierStokesStepper(Stepper):
         nx, ny, nz = field_extents(f_0)
         q, ext = self.velocity_set.q, (nx, ny, nz)
         expect_field(f_0, q, ext, "f_0", self.store_dtype)
-        expect_field(f_1, q, ext, "f_1", self.store_dtype)
-        if f_0.data_ptr() == f_1.data_ptr():
-            raise ValueError("f_0 and f_1 must be distinct buffers")
+        if f_1 is not None:
+            expect_field(f_1, q, ext, "f_1", self.store_dtype)
+            if f_0.data_ptr() == f_1.data_ptr():
+                raise ValueError("f_0 and f_1 must be distinct buffers")
         expect_field(bc_mask, 1, ext, "bc_mask", torch.uint8)
         self._sync_state(bc_mask, missing_mask)
         force_mode, force_vec, force_field = self._force_args()
@@ -208,6 +211,33 @@ class IncompressibleNavierStokesStepper(Stepper):
         )
         _lib.mark_written(f_1)
         return f_0, f_1
+
+    def adjoint_step(self, f_t, lam_out, lam_in, bc_mask, missing_mask, omega, grad_force=None):
+        """Reverse mode of one step (xlbh_adjoint_step): given the state f_t a forward step read and
+        lam_out = dL/df_{t+1}, write lam_in = dL/df_t and add dL/dg to ``grad_force`` ((d, nx, ny, nz)
+        float64) when the stepper has a body force.  BGK only."""
+        if self.collision_type != "BGK":
+            raise NotImplementedError("the adjoint step is implemented for BGK collision")
+        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_t, None, bc_mask, missing_mask)
+        q, ext = self.velocity_set.q, (nx, ny, nz)
+        expect_field(lam_out, q, ext, "lam_out", self.compute_dtype)
+        expect_field(lam_in, q, ext, "lam_in", self.compute_dtype)
+        if lam_in.data_ptr() == lam_out.data_ptr():
+            raise ValueError("lam_in and lam_out must be distinct buffers")
+        if grad_force is not None:
+            expect_field(grad_force, self.velocity_set.d, ext, "grad_force", torch.float64)
+        if self._has_solid:
+            raise NotImplementedError("the adjoint step does not support solid (255) cells")
+        _lib.check(
+            _lib.lib().xlbh_adjoint_step(
+                self.lattice_id, self.precision_policy.store_format, self.compute_id,
+                _lib.ptr(f_t), _lib.ptr(lam_out), _lib.ptr(lam_in), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
+                _lib.ptr(grad_force), nx, ny, nz, float(omega), _lib.stream_of(f_t),
+            ),
+            "IncompressibleNavierStokesStepper.adjoint_step",
+        )
+        _lib.mark_written(lam_in, grad_force)
+        return lam_in
 
     def run(self, f_0, f_1, bc_mask, missing_mask, omega, num_steps, timestep=0):
         """``num_steps`` steps with the buffer swap in C; returns (f_0, f_1) with the newest
