@@ -1,22 +1,36 @@
 """Benchmark of the fused LBM step on one MI355X.
 
-Workload (the reference's own benchmark, examples/performance/mlups_3d.py:45-90, at edge 256):
-D3Q19 BGK lid-driven cavity 256^3, rho = 1 / u = 0 equilibrium start, lid EquilibriumBC(rho=1,
-u=(0.02, 0, 0)) on the top face without edges, FullwayBounceBackBC on the other five full faces,
-omega = 1.0.  One step = one fused stream-collide-BC pass over the grid; the timed loop runs
-K steps with the buffer swap below the C ABI (xlbh_run), all inputs resident in HBM.
+Headline workload (the reference's own benchmark, examples/performance/mlups_3d.py:45-90, at
+edge 256): D3Q19 BGK lid-driven cavity 256^3, rho = 1 / u = 0 equilibrium start, lid
+EquilibriumBC(rho=1, u=(0.02, 0, 0)) on the top face without edges, FullwayBounceBackBC on the
+other five full faces, omega = 1.0.  One step = one fused stream-collide-BC pass over the grid;
+the timed loop runs K steps with the buffer swap below the C ABI (xlbh_run), all inputs resident
+in HBM.  value = MLUPS = 256^3 K / t / 1e6 with the FP32/FP32 policy.
 
-value = MLUPS = 256^3 * K / t / 1e6 with the FP32/FP32 policy.  The same workload is also timed
-with 16-bit population storage (plain FP16, and f - w_l shifted FP16 / BF16) and reported under
-"store16".  roofline.achieved = algorithmic bytes per launch (2 Q S_store per cell) / average
-kernel duration from HIP events on the launch stream.  cpu_baseline = the C++/OpenMP oracle
-(the parity checker, a restatement of the reference algorithm) timed on a bounded number of
-steps of the same workload on the host cores.
+Also reported on the same line:
+* store16 -- the same workload with 16-bit population storage (plain FP16, and f - w_l shifted
+  FP16 / BF16), FP32 arithmetic;
+* precision -- periodic D3Q19 BGK Taylor-Green vortex 512x256x256, u0 = 0.04, nu = 0.01, 2000
+  steps: relative L2 velocity error and kinetic-energy decay of every storage format against an
+  FP64/FP64 run of the same kernel (itself matched to the FP64 CPU oracle to 1e-12 by the tests);
+* adjoint -- backward (checkpoint recompute + adjoint kernels) over forward time per step for the
+  differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50) and the 256^3 cavity;
+* kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, with the momentum-exchange drag
+  every 100 steps inside the timed loop.
+roofline.achieved = algorithmic bytes per launch (2 Q S_store per cell) / average kernel duration
+from HIP events on the launch stream; roofline.traffic = FETCH_SIZE + WRITE_SIZE per launch from
+rocprofv3 (profiles/traffic_step_d3q19_cavity256.json, written by --summarize-pmc).
+cpu_baseline = the C++/OpenMP oracle (the parity checker, a restatement of the reference
+algorithm) timed on a bounded number of steps of the headline workload on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu]
+Usage:
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--quick]
+  python bench.py --profile [--steps K]        # headline kernel + a 1 GiB copy, for rocprofv3
+  python bench.py --summarize-pmc FETCH.csv WRITE.csv   # -> profiles/traffic_step_d3q19_cavity256.json
 """
 
 import argparse
+import csv
 import json
 import os
 import sys
@@ -29,73 +43,226 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MLUPS & % HBM roofline, D3Q19 BGK 256³ (FP32 vs FP16-store); adjoint/fwd time"
 HBM_PEAK_GBS = 8000.0
+COPY_CALIBRATION_BYTES = 1 << 30
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_step_d3q19_cavity{n}.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cavity_bcs(xa, grid):
-    box = grid.bounding_box_indices()
-    box_no_edge = grid.bounding_box_indices(remove_edges=True)
-    lid = box_no_edge["top"]
-    walls = [box["bottom"][i] + box["left"][i] + box["right"][i] + box["front"][i] + box["back"][i] for i in range(3)]
-    walls = np.unique(np.array(walls), axis=-1).tolist()
-    BC = xa.operator.boundary_condition
-    return [BC.EquilibriumBC(rho=1.0, u=(0.02, 0.0, 0.0), indices=lid), BC.FullwayBounceBackBC(indices=walls)]
+def store_bytes(torch, t):
+    return torch.empty((), dtype=t.dtype).element_size()
 
 
-def gpu_leg(xa, torch, policy, n, steps, warmup):
-    vs = xa.velocity_set.D3Q19(policy, xa.ComputeBackend.MI355X)
-    xa.init(velocity_set=vs, default_backend=xa.ComputeBackend.MI355X, default_precision_policy=policy)
-    xa.operator.boundary_condition.boundary_condition_registry.reset()
-    grid = xa.grid.grid_factory((n, n, n))
-    stepper = xa.operator.stepper.IncompressibleNavierStokesStepper(grid, cavity_bcs(xa, grid), collision_type="BGK")
-    f0, f1, bc_mask, missing = stepper.prepare_fields()
-    omega = 1.0
-    f0, f1 = stepper.run(f0, f1, bc_mask, missing, omega, warmup)
+def time_run(torch, wl, steps, warmup):
+    """Warm up, then time ``steps`` steps of ``stepper.run`` with HIP events on the launch stream
+    and the host clock; returns (wall_s, event_s, f_0, f_1)."""
+    st = wl.stepper
+    f0, f1 = st.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    f0, f1 = stepper.run(f0, f1, bc_mask, missing, omega, steps)
+    f0, f1 = st.run(f0, f1, wl.bc_mask, wl.missing_mask, wl.omega, steps, timestep=warmup)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ev_ms = ev0.elapsed_time(ev1)
+    return wall, ev0.elapsed_time(ev1) / 1e3, f0, f1
+
+
+def headline_leg(xa, torch, policy, n, steps, warmup, python_loop=True):
+    from xlb_amd.workloads import lid_driven_cavity_3d
+
+    wl = lid_driven_cavity_3d(n, policy)
+    wall, ev_s, f0, f1 = time_run(torch, wl, steps, warmup)
     cells = n**3
-    # Python-loop variant through the public __call__ (the reference's loop, mlups_3d.py:78-80)
-    py_steps = min(steps, 200)
+    res = dict(policy=policy.name, mlups=cells * steps / wall / 1e6, mlups_events=cells * steps / ev_s / 1e6,
+               ms_per_step=wall / steps * 1e3)
+    if python_loop:
+        # the reference's loop through the public __call__ (mlups_3d.py:78-80)
+        py_steps = min(steps, 200)
+        st = wl.stepper
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(py_steps):
+            f0, f1 = st(f0, f1, wl.bc_mask, wl.missing_mask, wl.omega, i)
+            f0, f1 = f1, f0
+        torch.cuda.synchronize()
+        res["mlups_python_loop"] = cells * py_steps / (time.perf_counter() - t1) / 1e6
+    res["finite"] = bool(torch.isfinite(f0.float()).all().item())
+    bpl = 2 * wl.stepper.velocity_set.q * store_bytes(torch, f0)
+    kernel_s = ev_s / steps
+    res.update(bytes_per_lu=bpl, kernel_ms=kernel_s * 1e3, achieved_gbs=bpl * cells / kernel_s / 1e9)
+    del wl, f0, f1
+    torch.cuda.empty_cache()
+    return res
+
+
+def decoded_velocity(torch, f, policy, vs, chunk=32):
+    """rho-weighted moments of stored populations in float64: returns u (d, ...) float64."""
+    w = torch.tensor(np.asarray(vs.w, dtype=np.float64), device=f.device).reshape(-1, 1, 1, 1)
+    c = torch.tensor(np.asarray(vs.c, dtype=np.float64), device=f.device)
+    u = torch.empty((vs.d,) + tuple(f.shape[1:]), dtype=torch.float64, device=f.device)
+    for i in range(0, f.shape[1], chunk):
+        fx = f[:, i:i + chunk].double()
+        if policy.shifted:
+            fx += w
+        rho = fx.sum(0)
+        u[:, i:i + chunk] = torch.tensordot(c, fx, dims=([1], [0])) / rho
+    return u
+
+
+def precision_leg(xa, torch, shape=(512, 256, 256), steps=2000):
+    """Taylor-Green vortex: every storage format against the FP64/FP64 run."""
+    from xlb_amd.workloads import taylor_green_3d
+
+    P = xa.PrecisionPolicy
+    out, u64, ke0 = {}, None, None
+    cells = int(np.prod(shape))
+    for pol in (P.FP64FP64, P.FP32FP32, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16, P.FP32BF16):
+        wl = taylor_green_3d(shape, pol)
+        vs = wl.stepper.velocity_set
+        if ke0 is None:
+            ke0 = float(0.5 * decoded_velocity(torch, wl.f_0, pol, vs).pow(2).sum(0).mean())
+        wall, ev_s, f0, f1 = time_run(torch, wl, steps, 0)
+        u = decoded_velocity(torch, f0, pol, vs)
+        ke = float(0.5 * u.pow(2).sum(0).mean())
+        rec = dict(mlups=cells * steps / ev_s / 1e6, ke_ratio=ke / ke0)
+        if u64 is None:
+            u64 = u
+        else:
+            rec["rel_l2_u_vs_fp64"] = float((u - u64).norm() / u64.norm())
+            rec["max_abs_u_vs_fp64"] = float((u - u64).abs().max())
+        out[pol.name] = rec
+        log(f"bench: precision {pol.name}: " + ", ".join(f"{k}={v:.3e}" for k, v in rec.items()))
+        del wl, f0, f1, u
+        torch.cuda.empty_cache()
+    return dict(workload=f"periodic D3Q19 BGK Taylor-Green vortex {shape[0]}x{shape[1]}x{shape[2]}, u0=0.04, "
+                         f"nu=0.01, {steps} steps; errors against the FP64/FP64 run", formats=out)
+
+
+def _sync_time(torch, fn):
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for i in range(py_steps):
-        f0, f1 = stepper(f0, f1, bc_mask, missing, omega, i)
-        f0, f1 = f1, f0
+    t0 = time.perf_counter()
+    r = fn()
     torch.cuda.synchronize()
-    py_wall = time.perf_counter() - t1
-    finite = bool(torch.isfinite(f0.float()).all().item())
-    store_bytes = torch.empty((), dtype=f0.dtype).element_size()
-    bytes_per_lu = 2 * vs.q * store_bytes
-    kernel_s = ev_ms / 1e3 / steps
-    res = dict(
-        policy=policy.name,
-        mlups=cells * steps / wall / 1e6,
-        mlups_events=cells * steps / (ev_ms / 1e3) / 1e6,
-        mlups_python_loop=cells * py_steps / py_wall / 1e6,
-        ms_per_step=wall / steps * 1e3,
-        kernel_ms=kernel_s * 1e3,
-        bytes_per_lu=bytes_per_lu,
-        achieved_gbs=bytes_per_lu * cells / kernel_s / 1e9,
-        finite=finite,
-    )
-    del f0, f1, bc_mask, missing, stepper
+    return time.perf_counter() - t0, r
+
+
+def adjoint_leg(xa, torch, quick=False):
+    """Backward / forward time per step of the checkpointed adjoint rollout."""
+    from xlb_amd.autodiff import macroscopic_torch, rollout
+    from xlb_amd.workloads import differentiable_channel_2d, lid_driven_cavity_3d
+
+    res = {}
+    # differentiable channel: loss = mean squared deviation of u at x = 480 from a seeded target
+    steps, every = (200, 50) if quick else (800, 50)
+    wl = differentiable_channel_2d((512, 256), xa.PrecisionPolicy.FP32FP32, seed=0)
+    st, g = wl.stepper, wl.extra["force_field"]
+    vs = st.velocity_set
+    rng = np.random.default_rng(1)
+    yy = np.arange(256, dtype=np.float64)
+    target = np.zeros((2, 256))
+    target[0] = 4 * 0.05 * yy * (255 - yy) / 255**2 * (1 + 0.05 * rng.standard_normal(256))
+    target = torch.from_numpy(target).cuda()
+
+    def loss(fT):
+        _, u = macroscopic_torch(fT[:, 480:481].double(), vs)
+        return ((u[:, 0, :, 0] - target) ** 2).mean()
+
+    def fwd_loop():
+        a, b = wl.f_0.clone(), wl.f_1.clone()
+        for t in range(steps):
+            st(a, b, wl.bc_mask, wl.missing_mask, wl.omega, t)
+            a, b = b, a
+        return a
+
+    fwd_loop()
+    t_fwd, _ = _sync_time(torch, fwd_loop)
+    t_run, _ = _sync_time(torch, lambda: st.run(wl.f_0.clone(), wl.f_1.clone(), wl.bc_mask, wl.missing_mask, wl.omega, steps))
+    gg = g.clone().requires_grad_(True)
+    t_roll, fT = _sync_time(torch, lambda: rollout(st, wl.f_0, wl.bc_mask, wl.missing_mask, wl.omega, steps,
+                                                    force_field=gg, checkpoint_every=every))
+    L = loss(fT)
+    t_bwd, _ = _sync_time(torch, lambda: L.backward())
+    res["channel2d"] = dict(
+        workload=f"D2Q9 BGK channel 512x256, Zou-He parabolic inlet u=0.05, extrapolation outlet, halfway walls, "
+                 f"per-cell force field, {steps} steps, checkpoint every {every}, FP32",
+        fwd_ms_per_step=t_fwd / steps * 1e3, fwd_run_ms_per_step=t_run / steps * 1e3,
+        rollout_fwd_ms_per_step=t_roll / steps * 1e3, bwd_ms_per_step=t_bwd / steps * 1e3,
+        ratio_bwd_over_fwd=t_bwd / t_fwd, grad_norm=float(gg.grad.norm()))
+    log(f"bench: adjoint channel2d: " + ", ".join(f"{k}={v:.4g}" for k, v in res["channel2d"].items() if k != "workload"))
+    del wl, st, gg, fT, L
+    torch.cuda.empty_cache()
+
+    # headline cavity 256^3: loss on one z-slice, sqrt-N checkpoints
+    n, steps = (128, 32) if quick else (256, 64)
+    wl = lid_driven_cavity_3d(n, xa.PrecisionPolicy.FP32FP32)
+    st = wl.stepper
+    st.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, 20)
+
+    def fwd_loop3():
+        a, b = wl.f_0.clone(), wl.f_1.clone()
+        for t in range(steps):
+            st(a, b, wl.bc_mask, wl.missing_mask, wl.omega, t)
+            a, b = b, a
+        return a
+
+    fwd_loop3()
+    t_fwd, _ = _sync_time(torch, fwd_loop3)
+    f0 = wl.f_0.clone().requires_grad_(True)
+    t_roll, fT = _sync_time(torch, lambda: rollout(st, f0, wl.bc_mask, wl.missing_mask, wl.omega, steps))
+    L = (fT[:, :, :, n // 2].double() ** 2).sum()
+    t_bwd, _ = _sync_time(torch, lambda: L.backward())
+    res["cavity3d"] = dict(
+        workload=f"D3Q19 BGK cavity {n}^3 (headline setup), {steps} steps, checkpoint every "
+                 f"{int(np.ceil(np.sqrt(steps)))}, FP32",
+        fwd_ms_per_step=t_fwd / steps * 1e3, rollout_fwd_ms_per_step=t_roll / steps * 1e3,
+        bwd_ms_per_step=t_bwd / steps * 1e3, ratio_bwd_over_fwd=t_bwd / t_fwd)
+    log(f"bench: adjoint cavity3d: " + ", ".join(f"{k}={v:.4g}" for k, v in res["cavity3d"].items() if k != "workload"))
+    del wl, st, f0, fT, L
+    torch.cuda.empty_cache()
+    return res
+
+
+def kbc_sphere_leg(xa, torch, steps=500, quick=False):
+    """D3Q27 KBC wind tunnel with the drag every 100 steps inside the timed loop."""
+    from xlb_amd.workloads import wind_tunnel_sphere
+
+    shape = (128, 64, 64) if quick else (256, 128, 128)
+    radius, center = (6, (32, 32, 32)) if quick else (12, (64, 64, 64))
+    wl = wind_tunnel_sphere(shape, xa.PrecisionPolicy.FP32FP32, radius, center)
+    st = wl.stepper
+    mt = xa.operator.force.MomentumTransfer(wl.extra["sphere_bc"])
+    f0, f1 = st.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, 100)
+    torch.cuda.synchronize()
+    drags = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(100, steps - done)
+        f0, f1 = st.run(f0, f1, wl.bc_mask, wl.missing_mask, wl.omega, k, timestep=100 + done)
+        drags.append(mt.force_tensor(f0, f1, wl.bc_mask, wl.missing_mask))
+        done += k
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = int(np.prod(shape))
+    bpl = 2 * 27 * 4
+    mlups = cells * steps / wall / 1e6
+    res = dict(workload=f"D3Q27 KBC wind tunnel {shape[0]}x{shape[1]}x{shape[2]}, sphere r={radius}, Re=200, "
+                        f"{steps} steps with momentum-exchange drag every 100, FP32",
+               mlups=mlups, achieved_gbs=mlups * 1e6 * bpl / 1e9, frac=mlups * 1e6 * bpl / 1e9 / HBM_PEAK_GBS,
+               drag_last=[float(v) for v in drags[-1].cpu()], finite=bool(torch.isfinite(f0).all().item()))
+    log(f"bench: kbc sphere {mlups:.0f} MLUPS ({res['achieved_gbs']:.0f} GB/s), drag {res['drag_last']}")
+    del wl, st, f0, f1
     torch.cuda.empty_cache()
     return res
 
 
 def cpu_leg(n, budget_s=12.0, max_steps=40):
-    """The C++/OpenMP oracle on the same workload, FP32: a bounded sample of steps."""
+    """The C++/OpenMP oracle on the headline workload, FP32: a bounded sample of steps."""
     from oracle import oracle as orc
 
     threads = len(os.sched_getaffinity(0))
@@ -129,11 +296,74 @@ def cpu_leg(n, budget_s=12.0, max_steps=40):
 
 
 def load_traffic(n):
-    path = os.path.join(ROOT, "profiles", f"traffic_step_d3q19_cavity{n}.json")
+    path = TRAFFIC_JSON.format(n=n)
     if os.path.exists(path):
         with open(path) as fh:
             return json.load(fh).get("bytes_per_launch")
     return None
+
+
+def profile_pass(xa, torch, n, steps):
+    """What the rocprofv3 runs profile: one 1 GiB device copy (the counter calibration), then
+    ``steps`` headline steps."""
+    a = torch.ones(COPY_CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    r = headline_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n, steps, 5, python_loop=False)
+    log(f"bench: profile pass {r['mlups']:.0f} MLUPS, kernel {r['kernel_ms']:.4f} ms")
+
+
+def _read_pmc(path, counter):
+    """[(dispatch id, kernel name, value)] of one counter from a rocprofv3 counter_collection.csv,
+    in dispatch order."""
+    rows = []
+    with open(path, newline="") as fh:
+        for row in csv.DictReader(fh):
+            if row.get("Counter_Name") == counter:
+                rows.append((int(row["Dispatch_Id"]), row["Kernel_Name"], float(row["Counter_Value"])))
+    return sorted(rows)
+
+
+def summarize_pmc(fetch_csv, write_csv, n):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the step kernel, each scaled by what the same
+    counter reads for the 1 GiB calibration copy (the first three copy dispatches of the profiled
+    process, issued by ``profile_pass`` before anything else is copied) ->
+    profiles/traffic_step_d3q19_cavity{n}.json."""
+    fetch, write = _read_pmc(fetch_csv, "FETCH_SIZE"), _read_pmc(write_csv, "WRITE_SIZE")
+
+    def calib(rows):
+        vals = [v for _, k, v in rows if "copy" in k.lower() and "xlbh" not in k][:3]
+        if len(vals) != 3:
+            raise SystemExit("calibration copy dispatches not found")
+        return float(np.median(vals))
+
+    def step(rows):
+        ks = sorted({k for _, k, _ in rows if "k_step<" in k})
+        if len(ks) != 1:
+            raise SystemExit(f"expected one step kernel, found {ks}")
+        return ks[0], float(np.median([v for _, k, v in rows if k == ks[0]]))
+
+    copy_fetch, copy_write = calib(fetch), calib(write)
+    kernel, step_fetch = step(fetch)
+    _, step_write = step(write)
+    kib = COPY_CALIBRATION_BYTES / 1024.0
+    f_scale, w_scale = kib / copy_fetch, kib / copy_write
+    traffic = (step_fetch * f_scale + step_write * w_scale) * 1024.0
+    algorithmic = 2 * 19 * 4 * n**3
+    out = dict(kernel=kernel, copy_bytes=COPY_CALIBRATION_BYTES,
+               copy_fetch_kib=copy_fetch, copy_write_kib=copy_write, fetch_scale=f_scale, write_scale=w_scale,
+               step_fetch_kib=step_fetch, step_write_kib=step_write,
+               fetch_bytes=step_fetch * f_scale * 1024.0, write_bytes=step_write * w_scale * 1024.0,
+               bytes_per_launch=traffic, algorithmic_bytes_per_launch=algorithmic,
+               traffic_over_algorithmic=traffic / algorithmic)
+    path = TRAFFIC_JSON.format(n=n)
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+    return out
 
 
 def main():
@@ -144,7 +374,14 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-store16", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the precision, adjoint and KBC legs")
+    ap.add_argument("--quick", action="store_true", help="smaller extra legs")
+    ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.size)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
@@ -153,16 +390,25 @@ def main():
     import xlb_amd as xa
 
     n = args.size
+    if args.profile:
+        profile_pass(xa, torch, n, args.steps)
+        return
     log(f"bench: fp32 leg, {n}^3, warmup {args.warmup}, steps {args.steps}")
-    fp32 = gpu_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n, args.steps, args.warmup)
+    fp32 = headline_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n, args.steps, args.warmup)
     log(f"bench: fp32 {fp32['mlups']:.0f} MLUPS ({fp32['achieved_gbs']:.0f} GB/s)")
     store16 = {}
     if not args.no_store16:
         for pol in (xa.PrecisionPolicy.FP32FP16, xa.PrecisionPolicy.FP32FP16_SHIFTED, xa.PrecisionPolicy.FP32BF16_SHIFTED):
-            r = gpu_leg(xa, torch, pol, n, args.steps, args.warmup)
+            r = headline_leg(xa, torch, pol, n, args.steps, args.warmup, python_loop=False)
             log(f"bench: {pol.name} {r['mlups']:.0f} MLUPS ({r['achieved_gbs']:.0f} GB/s)")
             store16[pol.name] = dict(mlups=r["mlups"], achieved_gbs=r["achieved_gbs"], kernel_ms=r["kernel_ms"],
                                      speedup_vs_fp32=r["mlups"] / fp32["mlups"], frac=r["achieved_gbs"] / HBM_PEAK_GBS)
+    extras = {}
+    if not args.no_extras:
+        pshape = (128, 64, 64) if args.quick else (512, 256, 256)
+        extras["precision"] = precision_leg(xa, torch, pshape, 200 if args.quick else 2000)
+        extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
+        extras["kbc_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline (oracle) ...")
@@ -208,6 +454,7 @@ def main():
         "store16": store16,
         "finite": fp32["finite"],
     }
+    out.update(extras)
     print(json.dumps(out))
 
 
