@@ -101,16 +101,24 @@ def headline_leg(xa, torch, policy, n, steps, warmup, python_loop=True):
 
 
 def decoded_velocity(torch, f, policy, vs, chunk=32):
-    """rho-weighted moments of stored populations in float64: returns u (d, ...) float64."""
-    w = torch.tensor(np.asarray(vs.w, dtype=np.float64), device=f.device).reshape(-1, 1, 1, 1)
-    c = torch.tensor(np.asarray(vs.c, dtype=np.float64), device=f.device)
+    """Velocity (d, ...) in float64 of stored populations, by explicit sums over the directions in
+    x-chunks (no BLAS call: torch's tensordot on fields of this size, or on strided views into
+    them, returned wrong moments on ROCm)."""
+    c = np.asarray(vs.c, dtype=np.float64)
+    w = np.asarray(vs.w, dtype=np.float64)
     u = torch.empty((vs.d,) + tuple(f.shape[1:]), dtype=torch.float64, device=f.device)
     for i in range(0, f.shape[1], chunk):
-        fx = f[:, i:i + chunk].double()
-        if policy.shifted:
-            fx += w
-        rho = fx.sum(0)
-        u[:, i:i + chunk] = torch.tensordot(c, fx, dims=([1], [0])) / rho
+        rho = torch.zeros_like(u[0, i:i + chunk])
+        mom = torch.zeros_like(u[:, i:i + chunk])
+        for l in range(vs.q):
+            fl = f[l, i:i + chunk].double()
+            if policy.shifted:
+                fl = fl + w[l]
+            rho += fl
+            for d in range(vs.d):
+                if c[d, l] != 0:
+                    mom[d] += c[d, l] * fl
+        u[:, i:i + chunk] = mom / rho
     return u
 
 
@@ -188,7 +196,7 @@ def adjoint_leg(xa, torch, quick=False):
     L = loss(fT)
     t_bwd, _ = _sync_time(torch, lambda: L.backward())
     res["channel2d"] = dict(
-        workload=f"D2Q9 BGK channel 512x256, Zou-He parabolic inlet u=0.05, extrapolation outlet, halfway walls, "
+        workload=f"D2Q9 BGK channel 512x256, regularized parabolic inlet u=0.05, extrapolation outlet, halfway walls, "
                  f"per-cell force field, {steps} steps, checkpoint every {every}, FP32",
         fwd_ms_per_step=t_fwd / steps * 1e3, fwd_run_ms_per_step=t_run / steps * 1e3,
         rollout_fwd_ms_per_step=t_roll / steps * 1e3, bwd_ms_per_step=t_bwd / steps * 1e3,
@@ -255,8 +263,79 @@ def kbc_sphere_leg(xa, torch, steps=500, quick=False):
                         f"{steps} steps with momentum-exchange drag every 100, FP32",
                mlups=mlups, achieved_gbs=mlups * 1e6 * bpl / 1e9, frac=mlups * 1e6 * bpl / 1e9 / HBM_PEAK_GBS,
                drag_last=[float(v) for v in drags[-1].cpu()], finite=bool(torch.isfinite(f0).all().item()))
+    res["cd_last"] = 2.0 * res["drag_last"][0] / (0.04**2 * np.pi * radius**2)
     log(f"bench: kbc sphere {mlups:.0f} MLUPS ({res['achieved_gbs']:.0f} GB/s), drag {res['drag_last']}")
     del wl, st, f0, f1
+    torch.cuda.empty_cache()
+    return res
+
+
+def periodic_leg(xa, torch, n, steps, warmup):
+    """The pure step kernel: D3Q19 BGK FP32 periodic box without boundary conditions."""
+    from xlb_amd.workloads import periodic_box_3d
+
+    wl = periodic_box_3d(n, xa.PrecisionPolicy.FP32FP32)
+    wall, ev_s, f0, f1 = time_run(torch, wl, steps, warmup)
+    cells = n**3
+    res = dict(workload=f"D3Q19 BGK FP32 periodic {n}^3, no boundary conditions", mlups=cells * steps / wall / 1e6,
+               kernel_ms=ev_s / steps * 1e3, achieved_gbs=152 * cells / (ev_s / steps) / 1e9)
+    res["frac"] = res["achieved_gbs"] / HBM_PEAK_GBS
+    del wl, f0, f1
+    torch.cuda.empty_cache()
+    return res
+
+
+def copy_leg(torch, nbytes=2 * 19 * 4 * 256**3 // 2, reps=20):
+    """Device-to-device copy rate (torch copy_ of one 1.27 GB buffer into another): the practical
+    streaming ceiling the step kernel is compared with."""
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    ev1.record()
+    torch.cuda.synchronize()
+    s = ev0.elapsed_time(ev1) / 1e3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return dict(bytes=2 * nbytes, gbs=2 * nbytes / s / 1e9)
+
+
+def c1_leg(xa, torch, with_cpu=True):
+    """C1: 64^3 lid-driven cavity, halfway walls, lid u = (0.05, 0, 0), Re = 100, 1000 steps, FP32:
+    GPU rate, and the oracle's rate on the host cores for the same 1000 steps."""
+    from xlb_amd.workloads import lid_driven_cavity_3d
+
+    n, steps = 64, 1000
+    omega = 1.0 / (3 * 0.05 * n / 100 + 0.5)
+    wl = lid_driven_cavity_3d(n, xa.PrecisionPolicy.FP32FP32, lid_u=0.05, omega=omega, walls="halfway")
+    wall, ev_s, f0, f1 = time_run(torch, wl, steps, 20)
+    res = dict(workload="lid-driven cavity D3Q19 BGK 64^3, halfway walls, lid u=0.05, Re=100, 1000 steps, FP32",
+               gpu_mlups=n**3 * steps / wall / 1e6)
+    if with_cpu:
+        from oracle import oracle as orc
+
+        threads = len(os.sched_getaffinity(0))
+        orc.set_threads(threads)
+        mask = wl.bc_mask.cpu().numpy()
+        miss = wl.missing_mask.cpu().numpy().astype(np.uint8)
+        bcs = [dict(id=int(bc.id), kind=int(bc.kind), rho=float(getattr(bc, "rho", 0.0)),
+                    u=tuple(float(v) for v in getattr(bc, "u", (0.0, 0.0, 0.0)))) for bc in wl.stepper.boundary_conditions]
+        a = np.zeros((19, n, n, n), np.float32)
+        orc.equilibrium(1, orc.F32, 0, np.ones((n, n, n)), np.zeros((3, n, n, n)), a)
+        b = a.copy()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            orc.step(1, 0, orc.F32, 0, a, b, mask, miss, bcs, omega)
+            a, b = b, a
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = dict(value=n**3 * steps / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
+                                   sample="all 1000 steps of C1, C++/OpenMP oracle, FP32")
+        res["gpu_over_cpu"] = res["gpu_mlups"] / res["cpu_baseline"]["value"]
+    del wl, f0, f1
     torch.cuda.empty_cache()
     return res
 
@@ -403,8 +482,14 @@ def main():
             log(f"bench: {pol.name} {r['mlups']:.0f} MLUPS ({r['achieved_gbs']:.0f} GB/s)")
             store16[pol.name] = dict(mlups=r["mlups"], achieved_gbs=r["achieved_gbs"], kernel_ms=r["kernel_ms"],
                                      speedup_vs_fp32=r["mlups"] / fp32["mlups"], frac=r["achieved_gbs"] / HBM_PEAK_GBS)
+    copy = copy_leg(torch)
+    log(f"bench: device copy {copy['gbs']:.0f} GB/s")
     extras = {}
     if not args.no_extras:
+        extras["periodic"] = periodic_leg(xa, torch, n, args.steps, args.warmup)
+        log(f"bench: periodic {extras['periodic']['mlups']:.0f} MLUPS ({extras['periodic']['achieved_gbs']:.0f} GB/s)")
+        extras["c1"] = c1_leg(xa, torch, with_cpu=not args.no_cpu)
+        log(f"bench: c1 " + ", ".join(f"{k}={v}" for k, v in extras["c1"].items() if k != "workload"))
         pshape = (128, 64, 64) if args.quick else (512, 256, 256)
         extras["precision"] = precision_leg(xa, torch, pshape, 200 if args.quick else 2000)
         extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
@@ -447,6 +532,8 @@ def main():
             "kernel": "xlbh::k_step<D3Q19, BGK, F32, float>",
             "algorithmic_bytes_per_cell": fp32["bytes_per_lu"],
             "kernel_ms": fp32["kernel_ms"],
+            "device_copy_gbs": copy["gbs"],
+            "frac_of_device_copy": fp32["achieved_gbs"] / copy["gbs"],
         },
         "cpu_baseline": cpu,
         "mlups_events": fp32["mlups_events"],

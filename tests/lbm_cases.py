@@ -225,6 +225,14 @@ CASES = {
 }
 
 
+# Full-size configurations, used by name only (not part of the all-case sweeps).
+LARGE_CASES = {
+    # C1: 64^3 lid-driven cavity, halfway walls, lid u = (0.05, 0, 0), Re = 100 (nu = 0.05*64/100)
+    "c1_cavity64": ("D3Q19", (64, 64, 64), "BGK", 1.0 / (3 * 0.05 * 64 / 100 + 0.5), "rest"),
+}
+BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway"}
+
+
 def random_init(vs, shape, seed=0, du=0.02):
     rng = np.random.default_rng(seed)
     rho = 1.0 + 0.01 * rng.standard_normal(shape)
@@ -239,8 +247,8 @@ class Pair:
         import xlb_amd as xa
         import torch
 
-        lat_name, shape, coll, omega, init = CASES[name]
-        bc_name = "sphere3d" if name == "sphere3d_bgk" else name
+        lat_name, shape, coll, omega, init = CASES[name] if name in CASES else LARGE_CASES[name]
+        bc_name = BC_SET.get(name, name)
         self.name, self.policy, self.omega, self.coll = name, policy, omega, coll
         vs = getattr(xa.velocity_set, lat_name)(policy, xa.ComputeBackend.MI355X)
         xa.init(velocity_set=vs, default_backend=xa.ComputeBackend.MI355X, default_precision_policy=policy)

@@ -84,6 +84,22 @@ def test_stepper_fp32_matches_oracle(name):
     compare(Pair(name, P.FP32FP32), 40)
 
 
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+def test_c1_cavity64_1000_steps_matches_oracle(policy):
+    """The 64^3 lid-driven cavity (halfway walls, lid u = 0.05, Re = 100) after 1000 steps:
+    max |drho|, |du| <= 1e-5 in FP32 with no noise-floor relaxation; 1e-12 in FP64."""
+    pair = Pair("c1_cavity64", policy)
+    pair.step_product(1000, use_run=True)
+    pair.step_oracle(1000)
+    fa, fb = pair.product_f(), pair.oracle_f()
+    ra, ua = moments(fa, pair.vs.c)
+    rb, ub = moments(fb, pair.vs.c)
+    dr, du, df = np.abs(ra - rb).max(), np.abs(ua - ub).max(), np.abs(fa - fb).max()
+    print(f"C1 {policy.name}: max|drho| {dr:.2e} max|du| {du:.2e} max|df| {df:.2e} (max|u| {np.abs(ub).max():.3f})")
+    bound = 1e-5 if policy == P.FP32FP32 else 1e-12
+    assert dr <= bound and du <= bound
+
+
 @pytest.mark.parametrize("name", ["periodic", "cavity_fullway", "channel2d_zouhe", "channel2d_regularized", "sphere3d",
                                   "periodic_field", "do_nothing"])
 def test_stepper_fp64_matches_oracle(name):

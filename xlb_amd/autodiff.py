@@ -26,21 +26,36 @@ from xlb_amd.operator.collision import ForcedCollision
 
 def equilibrium_torch(rho, u, velocity_set):
     """feq(rho, u) as differentiable torch ops (quadratic_equilibrium.py:18-25); rho (1,...),
-    u (d,...) -> (q,...)."""
-    c = torch.as_tensor(np.asarray(velocity_set.c, dtype=np.float64), dtype=u.dtype, device=u.device)
-    w = torch.as_tensor(np.asarray(velocity_set.w, dtype=np.float64), dtype=u.dtype, device=u.device)
-    shape = (-1,) + (1,) * (u.dim() - 1)
-    cu = 3.0 * torch.tensordot(c.T, u, dims=([1], [0]))
-    usqr = 1.5 * (u * u).sum(0, keepdim=True)
-    return rho * w.reshape(shape) * (1.0 + cu * (1.0 + 0.5 * cu) - usqr)
+    u (d,...) -> (q,...).  Sums over directions are explicit elementwise ops (no BLAS call, which
+    misbehaved on ROCm for fields of several GB)."""
+    c = np.asarray(velocity_set.c, dtype=np.float64)
+    w = np.asarray(velocity_set.w, dtype=np.float64)
+    usqr = 1.5 * (u * u).sum(0)
+    out = []
+    for l in range(velocity_set.q):
+        cu = None
+        for d in range(velocity_set.d):
+            if c[d, l] != 0:
+                term = float(c[d, l]) * u[d]
+                cu = term if cu is None else cu + term
+        if cu is None:
+            out.append(rho[0] * w[l] * (1.0 - usqr))
+        else:
+            cu = 3.0 * cu
+            out.append(rho[0] * w[l] * (1.0 + cu * (1.0 + 0.5 * cu) - usqr))
+    return torch.stack(out)
 
 
 def macroscopic_torch(f, velocity_set):
-    """rho, u of populations f (q, ...) as differentiable torch ops."""
-    c = torch.as_tensor(np.asarray(velocity_set.c, dtype=np.float64), dtype=f.dtype, device=f.device)
+    """rho (1, ...), u (d, ...) of populations f (q, ...) as differentiable torch ops."""
+    c = np.asarray(velocity_set.c, dtype=np.float64)
     rho = f.sum(0, keepdim=True)
-    u = torch.tensordot(c, f, dims=([1], [0])) / rho
-    return rho, u
+    mom = []
+    for d in range(velocity_set.d):
+        pos = [l for l in range(velocity_set.q) if c[d, l] > 0]
+        neg = [l for l in range(velocity_set.q) if c[d, l] < 0]
+        mom.append(f[pos].sum(0) - f[neg].sum(0))
+    return rho, torch.stack(mom) / rho
 
 
 class _Checkpoint:

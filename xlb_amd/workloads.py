@@ -48,6 +48,21 @@ def _eq_state(stepper, f, rho, u):
     return f
 
 
+def periodic_box_3d(n, policy, omega=1.0, seed=0):
+    """D3Q19 BGK periodic box without boundary conditions (the pure step kernel), started from a
+    seeded random near-equilibrium flow."""
+    vs, grid = _setup("D3Q19", policy, (n, n, n))
+    st = xa.operator.stepper.IncompressibleNavierStokesStepper(grid, [], collision_type="BGK")
+    f0, f1, bm, mm = st.prepare_fields()
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    u = (0.02 * torch.randn((3, n, n, n), generator=g, dtype=torch.float32)).to(f0.device)
+    rho = torch.ones((1, n, n, n), dtype=torch.float32, device=f0.device)
+    _eq_state(st, f0, rho, u)
+    del u, rho
+    f1.copy_(f0)
+    return Workload(st, f0, f1, bm, mm, omega)
+
+
 def lid_driven_cavity_3d(n, policy, lid_u=0.02, omega=1.0, walls="fullway"):
     """mlups_3d.py:45-90: EquilibriumBC lid (top face without edges), bounce-back on the other
     five full faces, rest start.  walls = "fullway" (as mlups_3d.py) or "halfway"."""
@@ -114,8 +129,9 @@ def taylor_green_3d(shape, policy, u0=0.04, nu=0.01):
 
 
 def differentiable_channel_2d(shape, policy, u_in=0.05, omega=1.6, force_std=1e-6, seed=0):
-    """D2Q9 BGK channel: parabolic Zou-He velocity inlet on the left, extrapolation outflow on the
-    right, halfway bounce-back on the top and bottom walls, and a learnable per-cell body-force
+    """D2Q9 BGK channel: parabolic regularized velocity inlet on the left (normal-speed profile as
+    flow_past_sphere_3d.py:85-97), extrapolation outflow on the right, halfway bounce-back on the
+    full top and bottom faces (lid_driven_cavity_2d.py:55), and a learnable per-cell body-force
     field g ~ N(0, force_std^2) (seeded) applied by the exact-difference scheme."""
     BC = xa.operator.boundary_condition
     vs, grid = _setup("D2Q9", policy, shape)
@@ -129,7 +145,7 @@ def differentiable_channel_2d(shape, policy, u_in=0.05, omega=1.6, force_std=1e-
 
     bcs = [
         BC.HalfwayBounceBackBC(indices=_faces(box, ["bottom", "top"], 2)),
-        BC.ZouHeBC("velocity", profile=parabola, indices=ne["left"]),
+        BC.RegularizedBC("velocity", profile=parabola, indices=ne["left"]),
         BC.ExtrapolationOutflowBC(indices=ne["right"]),
     ]
     rng = np.random.default_rng(seed)
