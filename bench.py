@@ -151,6 +151,30 @@ def precision_leg(xa, torch, shape=(512, 256, 256), steps=2000):
                          f"nu=0.01, {steps} steps; errors against the FP64/FP64 run", formats=out)
 
 
+def headline_precision_leg(xa, torch, n=256, steps=1000):
+    """Velocity error of every storage format on the headline cavity after ``steps`` steps,
+    against the FP64/FP64 run of the same workload."""
+    from xlb_amd.workloads import lid_driven_cavity_3d
+
+    P = xa.PrecisionPolicy
+    out, u64 = {}, None
+    for pol in (P.FP64FP64, P.FP32FP32, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16, P.FP32BF16):
+        wl = lid_driven_cavity_3d(n, pol)
+        f0, _ = wl.stepper.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, steps)
+        u = decoded_velocity(torch, f0, pol, wl.stepper.velocity_set)
+        if u64 is None:
+            u64 = u
+            out[pol.name] = dict(max_abs_u=float(u.abs().max()))
+        else:
+            out[pol.name] = dict(rel_l2_u_vs_fp64=float((u - u64).norm() / u64.norm()),
+                                 max_abs_u_vs_fp64=float((u - u64).abs().max()))
+        log(f"bench: cavity precision {pol.name}: " + ", ".join(f"{k}={v:.3e}" for k, v in out[pol.name].items()))
+        del wl, f0, u
+        torch.cuda.empty_cache()
+    return dict(workload=f"headline cavity D3Q19 BGK {n}^3 (lid u=0.02, omega=1.0), {steps} steps; errors against "
+                         "the FP64/FP64 run", formats=out)
+
+
 def _sync_time(torch, fn):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -160,77 +184,70 @@ def _sync_time(torch, fn):
 
 
 def adjoint_leg(xa, torch, quick=False):
-    """Backward / forward time per step of the checkpointed adjoint rollout."""
+    """Backward / forward time per step of the checkpointed adjoint rollout.  The forward reference
+    is the plain forward (``stepper.run``, the step loop below the ABI); the backward is the
+    whole reverse pass (checkpoint recompute + adjoint kernels, xlbh_adjoint_run per segment)."""
     from xlb_amd.autodiff import macroscopic_torch, rollout
     from xlb_amd.workloads import differentiable_channel_2d, lid_driven_cavity_3d
+
+    def measure(wl, steps, every, loss, grad_wrt_force):
+        st = wl.stepper
+        a0, b0 = wl.f_0.clone(), wl.f_1.clone()
+        st.run(a0, b0, wl.bc_mask, wl.missing_mask, wl.omega, steps)  # warm-up
+        a0.copy_(wl.f_0)
+        t_run, _ = _sync_time(torch, lambda: st.run(a0, b0, wl.bc_mask, wl.missing_mask, wl.omega, steps))
+        if grad_wrt_force:
+            x = wl.extra["force_field"].clone().requires_grad_(True)
+            args = dict(force_field=x)
+            f0 = wl.f_0
+        else:
+            x = wl.f_0.clone().requires_grad_(True)
+            args = {}
+            f0 = x
+        t_roll, fT = _sync_time(torch, lambda: rollout(st, f0, wl.bc_mask, wl.missing_mask, wl.omega, steps,
+                                                        checkpoint_every=every, **args))
+        L = loss(fT)
+        torch.cuda.synchronize()
+        t_bwd, _ = _sync_time(torch, lambda: L.backward())
+        return dict(fwd_ms_per_step=t_run / steps * 1e3, rollout_fwd_ms_per_step=t_roll / steps * 1e3,
+                    bwd_ms_per_step=t_bwd / steps * 1e3, ratio_bwd_over_fwd=t_bwd / t_run,
+                    ratio_fwd_plus_bwd_over_fwd=(t_roll + t_bwd) / t_run, grad_norm=float(x.grad.norm()))
 
     res = {}
     # differentiable channel: loss = mean squared deviation of u at x = 480 from a seeded target
     steps, every = (200, 50) if quick else (800, 50)
     wl = differentiable_channel_2d((512, 256), xa.PrecisionPolicy.FP32FP32, seed=0)
-    st, g = wl.stepper, wl.extra["force_field"]
-    vs = st.velocity_set
+    vs = wl.stepper.velocity_set
     rng = np.random.default_rng(1)
     yy = np.arange(256, dtype=np.float64)
     target = np.zeros((2, 256))
     target[0] = 4 * 0.05 * yy * (255 - yy) / 255**2 * (1 + 0.05 * rng.standard_normal(256))
     target = torch.from_numpy(target).cuda()
 
-    def loss(fT):
+    def channel_loss(fT):
         _, u = macroscopic_torch(fT[:, 480:481].double(), vs)
         return ((u[:, 0, :, 0] - target) ** 2).mean()
 
-    def fwd_loop():
-        a, b = wl.f_0.clone(), wl.f_1.clone()
-        for t in range(steps):
-            st(a, b, wl.bc_mask, wl.missing_mask, wl.omega, t)
-            a, b = b, a
-        return a
-
-    fwd_loop()
-    t_fwd, _ = _sync_time(torch, fwd_loop)
-    t_run, _ = _sync_time(torch, lambda: st.run(wl.f_0.clone(), wl.f_1.clone(), wl.bc_mask, wl.missing_mask, wl.omega, steps))
-    gg = g.clone().requires_grad_(True)
-    t_roll, fT = _sync_time(torch, lambda: rollout(st, wl.f_0, wl.bc_mask, wl.missing_mask, wl.omega, steps,
-                                                    force_field=gg, checkpoint_every=every))
-    L = loss(fT)
-    t_bwd, _ = _sync_time(torch, lambda: L.backward())
-    res["channel2d"] = dict(
-        workload=f"D2Q9 BGK channel 512x256, regularized parabolic inlet u=0.05, extrapolation outlet, halfway walls, "
-                 f"per-cell force field, {steps} steps, checkpoint every {every}, FP32",
-        fwd_ms_per_step=t_fwd / steps * 1e3, fwd_run_ms_per_step=t_run / steps * 1e3,
-        rollout_fwd_ms_per_step=t_roll / steps * 1e3, bwd_ms_per_step=t_bwd / steps * 1e3,
-        ratio_bwd_over_fwd=t_bwd / t_fwd, grad_norm=float(gg.grad.norm()))
-    log(f"bench: adjoint channel2d: " + ", ".join(f"{k}={v:.4g}" for k, v in res["channel2d"].items() if k != "workload"))
-    del wl, st, gg, fT, L
+    r = measure(wl, steps, every, channel_loss, True)
+    r["workload"] = (f"D2Q9 BGK channel 512x256, regularized parabolic inlet u=0.05, extrapolation outlet, halfway "
+                     f"walls, per-cell force field, {steps} steps, checkpoint every {every}, FP32; gradient w.r.t. the "
+                     "force field")
+    res["channel2d"] = r
+    log("bench: adjoint channel2d: " + ", ".join(f"{k}={v:.4g}" for k, v in r.items() if k != "workload"))
+    del wl
     torch.cuda.empty_cache()
 
-    # headline cavity 256^3: loss on one z-slice, sqrt-N checkpoints
+    # headline cavity: loss on one z-slice, sqrt-N checkpoints
     n, steps = (128, 32) if quick else (256, 64)
     wl = lid_driven_cavity_3d(n, xa.PrecisionPolicy.FP32FP32)
-    st = wl.stepper
-    st.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, 20)
-
-    def fwd_loop3():
-        a, b = wl.f_0.clone(), wl.f_1.clone()
-        for t in range(steps):
-            st(a, b, wl.bc_mask, wl.missing_mask, wl.omega, t)
-            a, b = b, a
-        return a
-
-    fwd_loop3()
-    t_fwd, _ = _sync_time(torch, fwd_loop3)
-    f0 = wl.f_0.clone().requires_grad_(True)
-    t_roll, fT = _sync_time(torch, lambda: rollout(st, f0, wl.bc_mask, wl.missing_mask, wl.omega, steps))
-    L = (fT[:, :, :, n // 2].double() ** 2).sum()
-    t_bwd, _ = _sync_time(torch, lambda: L.backward())
-    res["cavity3d"] = dict(
-        workload=f"D3Q19 BGK cavity {n}^3 (headline setup), {steps} steps, checkpoint every "
-                 f"{int(np.ceil(np.sqrt(steps)))}, FP32",
-        fwd_ms_per_step=t_fwd / steps * 1e3, rollout_fwd_ms_per_step=t_roll / steps * 1e3,
-        bwd_ms_per_step=t_bwd / steps * 1e3, ratio_bwd_over_fwd=t_bwd / t_fwd)
-    log(f"bench: adjoint cavity3d: " + ", ".join(f"{k}={v:.4g}" for k, v in res["cavity3d"].items() if k != "workload"))
-    del wl, st, f0, fT, L
+    wl.f_0, wl.f_1 = wl.stepper.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, 20)
+    every = int(np.ceil(np.sqrt(steps)))
+    r = measure(wl, steps, every, lambda fT: (fT[:, :, :, n // 2].double() ** 2).sum(), False)
+    r["workload"] = (f"D3Q19 BGK cavity {n}^3 (headline setup), {steps} steps, checkpoint every {every}, FP32; "
+                     "gradient w.r.t. the initial populations")
+    res["cavity3d"] = r
+    log("bench: adjoint cavity3d: " + ", ".join(f"{k}={v:.4g}" for k, v in r.items() if k != "workload"))
+    del wl
     torch.cuda.empty_cache()
     return res
 
@@ -492,6 +509,7 @@ def main():
         log(f"bench: c1 " + ", ".join(f"{k}={v}" for k, v in extras["c1"].items() if k != "workload"))
         pshape = (128, 64, 64) if args.quick else (512, 256, 256)
         extras["precision"] = precision_leg(xa, torch, pshape, 200 if args.quick else 2000)
+        extras["precision_headline"] = headline_precision_leg(xa, torch, 128 if args.quick else n, 1000)
         extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
         extras["kbc_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick)
     cpu = None

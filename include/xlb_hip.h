@@ -225,7 +225,8 @@ int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_
  *   lam_in    : written with dL/df_in, (Q, N) in the compute type (every element is written).
  *   grad_force: (D, N) FP64; dL/dg of the step is ADDED (FIELD and UNIFORM force modes; for the
  *               uniform vector sum the field).  May be NULL.
- * Other arguments as for xlbh_step.  Fields with solid cells (id 255) are not supported.
+ * Other arguments as for xlbh_step.  Solid cells (id 255), which the forward step leaves
+ * unwritten, pass nothing back.
  * ------------------------------------------------------------------------------------- */
 int xlbh_adjoint_step(int lattice, int store, int compute,
                       const void* f_in, const void* lam_out, void* lam_in,
@@ -233,6 +234,23 @@ int xlbh_adjoint_step(int lattice, int store, int compute,
                       const void* aux, const xlbh_bc_desc* bc_table,
                       int force_mode, const double* force_uniform, const void* force_field,
                       double* grad_force, int nx, int ny, int nz, double omega, void* stream);
+
+/* Reverse sweep over one checkpoint segment of n_steps BGK steps, below the ABI (the counterpart
+ * of xlbh_run for the backward pass; the reference replays a Warp tape per segment,
+ * autodiff_stepper_subroutine.py:459-466).
+ *   f_ckpt : the state f^{t0} at the segment start (store format), read only.
+ *   states : workspace of (n_steps - 1) fields (Q, N) in the store format; receives the
+ *            recomputed f^{t0+1} .. f^{t0+n_steps-1}.  May be NULL when n_steps == 1.
+ *   lam_a  : dL/df^{t0+n_steps} on entry.  lam_b: a second (Q, N) compute-type buffer.
+ * The result dL/df^{t0} is left in lam_a when n_steps is even and in lam_b when it is odd; the
+ * other buffer is overwritten.  grad_force accumulates as in xlbh_adjoint_step. */
+int xlbh_adjoint_run(int lattice, int store, int compute,
+                     const void* f_ckpt, void* states, void* lam_a, void* lam_b,
+                     const uint8_t* bc_mask, const uint32_t* missing_bits,
+                     const void* aux, const xlbh_bc_desc* bc_table,
+                     int force_mode, const double* force_uniform, const void* force_field,
+                     double* grad_force, int nx, int ny, int nz, double omega, int timestep0,
+                     int n_steps, void* stream);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,8 @@ def macroscopic_torch(f, velocity_set):
 
 
 class _Checkpoint:
+    """State at a segment start: kept in the store dtype, or as bfloat16 of f - w_l."""
+
     def __init__(self, f, mode, w):
         self.mode = mode
         if mode == "bf16":
@@ -66,12 +68,12 @@ class _Checkpoint:
         else:
             self.data = f.clone()
 
-    def restore(self, out, w):
+    def state(self, scratch, w):
+        """A store-dtype tensor holding the state (``scratch`` is filled for bf16 checkpoints)."""
         if self.mode == "bf16":
-            out.copy_(self.data.float() + w)
-        else:
-            out.copy_(self.data)
-        return out
+            scratch.copy_(self.data.float() + w)
+            return scratch
+        return self.data
 
 
 class _Rollout(torch.autograd.Function):
@@ -83,14 +85,12 @@ class _Rollout(torch.autograd.Function):
                 raise ValueError("a differentiable force field needs a stepper built with force_field=...")
             col.force_field = force_field.detach().to(stepper.compute_dtype).contiguous()
         w = torch.as_tensor(np.asarray(stepper.velocity_set.w), dtype=torch.float32, device=f0.device).reshape(-1, 1, 1, 1)
-        a = f0.detach().clone().contiguous()
+        a = f0.detach().to(stepper.store_dtype).clone().contiguous()
         b = torch.empty_like(a)
         ckpts = {}
-        for t in range(num_steps):
-            if t % every == 0:
-                ckpts[t] = _Checkpoint(a, ckpt_mode, w)
-            stepper(a, b, bc_mask, missing_mask, omega, t)
-            a, b = b, a
+        for s in range(0, num_steps, every):
+            ckpts[s] = _Checkpoint(a, ckpt_mode, w)
+            a, b = stepper.run(a, b, bc_mask, missing_mask, omega, min(every, num_steps - s), timestep=s)
         ctx.stepper = stepper
         ctx.masks = (bc_mask, missing_mask)
         ctx.omega = omega
@@ -101,31 +101,31 @@ class _Rollout(torch.autograd.Function):
         ctx.has_force = force_field is not None
         ctx.force_shape = None if force_field is None else tuple(force_field.shape)
         ctx.force_dtype = None if force_field is None else force_field.dtype
+        ctx.f0_dtype = f0.dtype
         return a
 
     @staticmethod
     def backward(ctx, grad_out):
         st = ctx.stepper
         bc_mask, missing_mask = ctx.masks
-        cdt = st.compute_dtype
-        lam = grad_out.detach().to(cdt).contiguous().clone()
-        lam_next = torch.empty_like(lam)
+        lam_a = grad_out.detach().to(st.compute_dtype).contiguous().clone()
+        lam_b = torch.empty_like(lam_a)
         grad_force = None
         if ctx.has_force:
-            grad_force = torch.zeros(ctx.force_shape, dtype=torch.float64, device=lam.device)
+            grad_force = torch.zeros(ctx.force_shape, dtype=torch.float64, device=lam_a.device)
         n, k = ctx.num_steps, ctx.every
-        starts = sorted(ctx.ckpts.keys())
-        seg_buf = [torch.empty(lam.shape, dtype=st.store_dtype, device=lam.device) for _ in range(min(k, n))]
-        for s in reversed(starts):
+        shape = tuple(lam_a.shape)
+        states = torch.empty((max(1, min(k, n) - 1),) + shape, dtype=st.store_dtype, device=lam_a.device)
+        scratch = torch.empty(shape, dtype=st.store_dtype, device=lam_a.device) if any(
+            c.mode == "bf16" for c in ctx.ckpts.values()) else None
+        for s in sorted(ctx.ckpts.keys(), reverse=True):
             length = min(k, n - s)
-            # recompute f_s .. f_{s+length-1}
-            ctx.ckpts[s].restore(seg_buf[0], ctx.w)
-            for j in range(1, length):
-                st(seg_buf[j - 1], seg_buf[j], bc_mask, missing_mask, ctx.omega, s + j - 1)
-            for j in reversed(range(length)):
-                st.adjoint_step(seg_buf[j], lam, lam_next, bc_mask, missing_mask, ctx.omega, grad_force)
-                lam, lam_next = lam_next, lam
-        g_f0 = lam.to(grad_out.dtype)
+            f_s = ctx.ckpts[s].state(scratch, ctx.w)
+            out = st.adjoint_run(f_s, states, lam_a, lam_b, bc_mask, missing_mask, ctx.omega, length, timestep=s,
+                                 grad_force=grad_force)
+            if out is lam_b:
+                lam_a, lam_b = lam_b, lam_a
+        g_f0 = lam_a.to(ctx.f0_dtype)
         g_force = grad_force.to(ctx.force_dtype) if grad_force is not None else None
         return g_f0, g_force, None, None, None, None, None, None, None
 

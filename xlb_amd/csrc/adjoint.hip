@@ -347,36 +347,7 @@ int launch_adjoint(const AdjParams& p, hipStream_t s) {
 
 using namespace xlbh;
 
-extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void* f_in, const void* lam_out,
-                                 void* lam_in, const uint8_t* bc_mask, const uint32_t* missing_bits,
-                                 const void* aux, const xlbh_bc_desc* bc_table, int force_mode,
-                                 const double* force_uniform, const void* force_field, double* grad_force, int nx,
-                                 int ny, int nz, double omega, void* stream) {
-  clear_error();
-  if (int e = check_grid(lattice, nx, ny, nz)) return e;
-  XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
-  XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
-  XLBH_REQUIRE(!bc_mask || (missing_bits && bc_table), "bc_mask given without missing bits or BC table");
-  XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
-  XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
-  XLBH_REQUIRE(force_mode != XLBH_FORCE_FIELD || force_field, "force field pointer is null");
-  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 31), "adjoint_step supports fewer than 2^31 cells");
-  AdjParams p;
-  p.f_in = f_in;
-  p.lam_out = lam_out;
-  p.lam_in = lam_in;
-  p.bc_mask = bc_mask;
-  p.missing = missing_bits;
-  p.aux = aux;
-  p.table = bc_table;
-  p.force_field = force_field;
-  p.grad_force = grad_force;
-  p.force_mode = force_mode;
-  for (int d = 0; d < 3; ++d) p.fu[d] = force_mode == XLBH_FORCE_UNIFORM ? force_uniform[d] : 0.0;
-  p.omega = omega;
-  p.sw = make_sweep(nx, ny, nz);
-  p.N = (uint32_t)((int64_t)nx * ny * nz);
-  hipStream_t s = (hipStream_t)stream;
+static int adjoint_dispatch(int lattice, int store, int compute, const AdjParams& p, hipStream_t s) {
   auto go = [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
     if (compute == XLBH_COMPUTE_F32) {
@@ -403,4 +374,92 @@ extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void
   }
   set_error("unknown lattice id");
   return 1;
+}
+
+static int adjoint_validate(int lattice, int force_mode, const double* force_uniform, const void* force_field,
+                            const uint8_t* bc_mask, const uint32_t* missing_bits, const xlbh_bc_desc* bc_table,
+                            int nx, int ny, int nz) {
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(!bc_mask || (missing_bits && bc_table), "bc_mask given without missing bits or BC table");
+  XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
+  XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
+  XLBH_REQUIRE(force_mode != XLBH_FORCE_FIELD || force_field, "force field pointer is null");
+  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 31), "the adjoint supports fewer than 2^31 cells");
+  return 0;
+}
+
+static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam_in, const uint8_t* bc_mask,
+                                const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
+                                int force_mode, const double* force_uniform, const void* force_field,
+                                double* grad_force, int nx, int ny, int nz, double omega) {
+  AdjParams p;
+  p.f_in = f_in;
+  p.lam_out = lam_out;
+  p.lam_in = lam_in;
+  p.bc_mask = bc_mask;
+  p.missing = missing_bits;
+  p.aux = aux;
+  p.table = bc_table;
+  p.force_field = force_field;
+  p.grad_force = grad_force;
+  p.force_mode = force_mode;
+  for (int d = 0; d < 3; ++d) p.fu[d] = force_mode == XLBH_FORCE_UNIFORM ? force_uniform[d] : 0.0;
+  p.omega = omega;
+  p.sw = make_sweep(nx, ny, nz);
+  p.N = (uint32_t)((int64_t)nx * ny * nz);
+  return p;
+}
+
+extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void* f_ckpt, void* states, void* lam_a,
+                                void* lam_b, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                                const xlbh_bc_desc* bc_table, int force_mode, const double* force_uniform,
+                                const void* force_field, double* grad_force, int nx, int ny, int nz, double omega,
+                                int timestep0, int n_steps, void* stream) {
+  clear_error();
+  if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
+                               ny, nz))
+    return e;
+  XLBH_REQUIRE(n_steps >= 1, "n_steps must be >= 1");
+  XLBH_REQUIRE(f_ckpt && lam_a && lam_b, "adjoint_run needs f_ckpt, lam_a and lam_b");
+  XLBH_REQUIRE(lam_a != lam_b, "lam_a must not alias lam_b");
+  XLBH_REQUIRE(n_steps == 1 || states, "adjoint_run needs a state workspace for n_steps > 1");
+  XLBH_REQUIRE(valid_store(store), "unknown store format");
+  const size_t field_bytes = (size_t)lattice_q(lattice) * (size_t)nx * ny * nz * store_bytes(store);
+  auto state = [&](int j) -> const void* {
+    return j == 0 ? f_ckpt : static_cast<const void*>(static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes);
+  };
+  // recompute f^{t0+1} .. f^{t0+n-1}
+  for (int j = 1; j < n_steps; ++j) {
+    if (int e = xlbh_step(lattice, XLBH_BGK, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
+                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega,
+                          timestep0 + j - 1, stream))
+      return e;
+  }
+  void* cur = lam_a;
+  void* nxt = lam_b;
+  for (int j = n_steps - 1; j >= 0; --j) {
+    const AdjParams p = adjoint_params(state(j), cur, nxt, bc_mask, missing_bits, aux, bc_table, force_mode,
+                                       force_uniform, force_field, grad_force, nx, ny, nz, omega);
+    if (int e = adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream)) return e;
+    void* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  return 0;
+}
+
+extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void* f_in, const void* lam_out,
+                                 void* lam_in, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                                 const void* aux, const xlbh_bc_desc* bc_table, int force_mode,
+                                 const double* force_uniform, const void* force_field, double* grad_force, int nx,
+                                 int ny, int nz, double omega, void* stream) {
+  clear_error();
+  if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
+                               ny, nz))
+    return e;
+  XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
+  XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
+  const AdjParams p = adjoint_params(f_in, lam_out, lam_in, bc_mask, missing_bits, aux, bc_table, force_mode,
+                                     force_uniform, force_field, grad_force, nx, ny, nz, omega);
+  return adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream);
 }

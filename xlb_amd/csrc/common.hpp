@@ -298,6 +298,8 @@ __device__ __forceinline__ int wrap(int v, int n) {
 
 inline bool valid_lattice(int lat) { return lat == XLBH_D2Q9 || lat == XLBH_D3Q19 || lat == XLBH_D3Q27; }
 inline bool valid_store(int s) { return s >= XLBH_F32 && s <= XLBH_BF16_SHIFTED; }
+inline int lattice_q(int lat) { return lat == XLBH_D2Q9 ? 9 : (lat == XLBH_D3Q19 ? 19 : 27); }
+inline size_t store_bytes(int s) { return s == XLBH_F64 ? 8 : (s == XLBH_F32 ? 4 : 2); }
 
 int check_grid(int lattice, int nx, int ny, int nz);
 

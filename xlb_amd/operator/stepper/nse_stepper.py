@@ -239,6 +239,42 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.mark_written(lam_in, grad_force)
         return lam_in
 
+    def adjoint_run(self, f_ckpt, states, lam_a, lam_b, bc_mask, missing_mask, omega, num_steps, timestep=0,
+                    grad_force=None):
+        """Reverse sweep over one checkpoint segment below the ABI (xlbh_adjoint_run): recompute the
+        ``num_steps - 1`` states after ``f_ckpt`` into ``states`` ((num_steps - 1, q, ...) in the
+        store dtype), then apply the adjoint steps in reverse starting from lam_a = dL/df at the
+        segment end.  Returns the buffer (lam_a or lam_b) that holds dL/df_ckpt.  BGK only."""
+        if self.collision_type != "BGK":
+            raise NotImplementedError("the adjoint step is implemented for BGK collision")
+        if self._has_solid:
+            raise NotImplementedError("the adjoint step does not support solid (255) cells")
+        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_ckpt, None, bc_mask, missing_mask)
+        q, ext = self.velocity_set.q, (nx, ny, nz)
+        expect_field(lam_a, q, ext, "lam_a", self.compute_dtype)
+        expect_field(lam_b, q, ext, "lam_b", self.compute_dtype)
+        if lam_a.data_ptr() == lam_b.data_ptr():
+            raise ValueError("lam_a and lam_b must be distinct buffers")
+        if num_steps < 1:
+            raise ValueError("num_steps must be >= 1")
+        if num_steps > 1:
+            if (states is None or states.dtype != self.store_dtype or not states.is_contiguous()
+                    or states.shape[0] < num_steps - 1 or tuple(states.shape[1:]) != tuple(f_ckpt.shape)):
+                raise ValueError("states must be a contiguous (num_steps - 1, q, nx, ny, nz) store-dtype tensor")
+        if grad_force is not None:
+            expect_field(grad_force, self.velocity_set.d, ext, "grad_force", torch.float64)
+        _lib.check(
+            _lib.lib().xlbh_adjoint_run(
+                self.lattice_id, self.precision_policy.store_format, self.compute_id,
+                _lib.ptr(f_ckpt), _lib.ptr(states), _lib.ptr(lam_a), _lib.ptr(lam_b), *masks, force_mode, fu_ptr,
+                _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, float(omega), int(timestep), int(num_steps),
+                _lib.stream_of(f_ckpt),
+            ),
+            "IncompressibleNavierStokesStepper.adjoint_run",
+        )
+        _lib.mark_written(states, lam_a, lam_b, grad_force)
+        return lam_a if num_steps % 2 == 0 else lam_b
+
     def run(self, f_0, f_1, bc_mask, missing_mask, omega, num_steps, timestep=0):
         """``num_steps`` steps with the buffer swap in C; returns (f_0, f_1) with the newest
         populations in f_0, i.e. the state the reference's loop
