@@ -125,6 +125,16 @@ int xlbh_run(int lattice, int collision, int store, int compute,
              int force_mode, const double* force_uniform, const void* force_field,
              int nx, int ny, int nz, double omega, int timestep0, int n_steps, void* stream);
 
+/* n_states steps from f_start, keeping every state: state j (the output of step t0 + j) is
+ * written to states + j * (Q N S_store) bytes, for j = 0 .. n_states - 1.  The forward pass of a
+ * checkpointed reverse sweep (xlbh_adjoint_run with recompute = 0); no counterpart in the
+ * reference, whose tape keeps the segment's states itself. */
+int xlbh_record(int lattice, int collision, int store, int compute, const void* f_start, void* states,
+                const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                const xlbh_bc_desc* bc_table, int force_mode, const double* force_uniform,
+                const void* force_field, int nx, int ny, int nz, double omega, int timestep0,
+                int n_states, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Standalone operators (the reference's per-operator Warp kernels).
  * ------------------------------------------------------------------------------------- */
@@ -225,6 +235,9 @@ int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_
  *   lam_in    : written with dL/df_in, (Q, N) in the compute type (every element is written).
  *   grad_force: (D, N) FP64; dL/dg of the step is ADDED (FIELD and UNIFORM force modes; for the
  *               uniform vector sum the field).  May be NULL.
+ *   bc_cells  : flat indices of every cell with a non-zero id, each once (xlbh_compact_cells with
+ *               bc_id = -1), so the boundary pass runs over them only; NULL (n_bc_cells = 0)
+ *               makes it scan the grid.  A list that misses boundary cells gives wrong gradients.
  * Other arguments as for xlbh_step.  Solid cells (id 255), which the forward step leaves
  * unwritten, pass nothing back.
  * ------------------------------------------------------------------------------------- */
@@ -232,6 +245,7 @@ int xlbh_adjoint_step(int lattice, int store, int compute,
                       const void* f_in, const void* lam_out, void* lam_in,
                       const uint8_t* bc_mask, const uint32_t* missing_bits,
                       const void* aux, const xlbh_bc_desc* bc_table,
+                      const int32_t* bc_cells, int n_bc_cells,
                       int force_mode, const double* force_uniform, const void* force_field,
                       double* grad_force, int nx, int ny, int nz, double omega, void* stream);
 
@@ -243,14 +257,17 @@ int xlbh_adjoint_step(int lattice, int store, int compute,
  *            recomputed f^{t0+1} .. f^{t0+n_steps-1}.  May be NULL when n_steps == 1.
  *   lam_a  : dL/df^{t0+n_steps} on entry.  lam_b: a second (Q, N) compute-type buffer.
  * The result dL/df^{t0} is left in lam_a when n_steps is even and in lam_b when it is odd; the
- * other buffer is overwritten.  grad_force accumulates as in xlbh_adjoint_step. */
+ * other buffer is overwritten.  grad_force and bc_cells as in xlbh_adjoint_step.  With
+ * recompute = 0 the states are taken as already recorded (xlbh_record), so the recompute of one
+ * segment can run on another stream while the reverse sweep of the next runs on this one. */
 int xlbh_adjoint_run(int lattice, int store, int compute,
                      const void* f_ckpt, void* states, void* lam_a, void* lam_b,
                      const uint8_t* bc_mask, const uint32_t* missing_bits,
                      const void* aux, const xlbh_bc_desc* bc_table,
+                     const int32_t* bc_cells, int n_bc_cells,
                      int force_mode, const double* force_uniform, const void* force_field,
                      double* grad_force, int nx, int ny, int nz, double omega, int timestep0,
-                     int n_steps, void* stream);
+                     int n_steps, int recompute, void* stream);
 
 #ifdef __cplusplus
 }

@@ -91,11 +91,11 @@ class Case:
         _, u = macroscopic_torch(fT, self.vs)
         return (self.W * fT).sum() + 50.0 * (u * u).sum()
 
-    def gradients(self, steps, every=None, ckpt="store"):
+    def gradients(self, steps, every=None, ckpt="store", overlap="auto"):
         f0 = self.f0.clone().requires_grad_(True)
         ff = self.ff.clone().requires_grad_(True) if self.ff is not None else None
         fT = rollout(self.st, f0, self.bc_mask, self.missing, self.omega, steps, force_field=ff,
-                     checkpoint_every=every, checkpoint_dtype=ckpt)
+                     checkpoint_every=every, checkpoint_dtype=ckpt, overlap=overlap)
         self.loss(fT).backward()
         return f0.grad.double(), (ff.grad.double() if ff is not None else None)
 
@@ -197,13 +197,15 @@ def test_checkpoint_spacing_invariance():
     steps = 13
     c = Case("channel2d_zouhe", P.FP64FP64, seed=4)
     ref_f, ref_g = c.gradients(steps, every=steps)
-    for every in (1, 3, None):
-        g_f, g_g = c.gradients(steps, every=every)
+    for every, overlap in ((1, True), (3, True), (None, True), (3, False), (None, False)):
+        g_f, g_g = c.gradients(steps, every=every, overlap=overlap)
         assert ((g_f - ref_f).norm() / ref_f.norm()).item() <= 1e-12
         assert ((g_g - ref_g).norm() / ref_g.norm()).item() <= 1e-12
     c32 = Case("channel2d_zouhe", P.FP32FP32, seed=4)
     e_f, e_g = c32.gradients(steps, every=4)
     b_f, b_g = c32.gradients(steps, every=4, ckpt="bf16")
+    b2_f, _ = c32.gradients(steps, every=4, ckpt="bf16", overlap=False)
+    assert ((b2_f - b_f).norm() / b_f.norm()).item() <= 1e-6
     r = ((b_f - e_f).norm() / e_f.norm()).item()
     assert r <= 1e-4, f"BF16 checkpoints: rel L2 {r:.3e}"
 
@@ -216,3 +218,28 @@ def test_adjoint_rejects_bad_arguments():
     kbc = xa.operator.stepper.IncompressibleNavierStokesStepper(c.st.grid, [], collision_type="KBC")
     with pytest.raises(NotImplementedError):
         kbc.adjoint_step(c.f0, lam, torch.zeros_like(lam), c.bc_mask, c.missing, c.omega)
+
+
+@pytest.mark.parametrize("name", ["tiny2d_channel", "tiny2d_regularized", "sphere3d", "do_nothing"])
+def test_boundary_pass_list_equals_grid_scan(name):
+    """The boundary pass over the compacted cell list (what the stepper passes) and over the whole
+    grid (bc_cells = NULL through the C ABI) give the same adjoint."""
+    import ctypes
+
+    from xlb_amd import _lib
+
+    c = Case(name, P.FP64FP64, seed=5)
+    st = c.st
+    lam_out = c.W.clone()
+    via_list = torch.empty_like(lam_out)
+    st.adjoint_step(c.f0, lam_out, via_list, c.bc_mask, c.missing, c.omega)
+    via_scan = torch.empty_like(lam_out)
+    nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
+    _lib.check(_lib.lib().xlbh_adjoint_step(
+        st.lattice_id, st.precision_policy.store_format, st.compute_id, _lib.ptr(c.f0), _lib.ptr(lam_out),
+        _lib.ptr(via_scan), *masks, None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz, float(c.omega),
+        _lib.stream_of(c.f0)))
+    torch.cuda.synchronize()
+    assert st._bc_cells is not None and st._bc_cells.numel() > 0
+    err = (via_list - via_scan).abs().max().item()
+    assert err <= 1e-12 * via_scan.abs().max().item()

@@ -71,9 +71,11 @@ SIGNATURES = {
     "xlbh_reduce_workspace_bytes": (_i64, [_i64]),
     "xlbh_momentum_transfer": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_mass_momentum": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
-    "xlbh_adjoint_step": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _vp, _i, _i, _i, _d, _vp]),
-    "xlbh_adjoint_run": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _vp, _i, _i, _i, _d,
-                              _i, _i, _vp]),
+    "xlbh_adjoint_step": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i, _i,
+                               _d, _vp]),
+    "xlbh_adjoint_run": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
+                              _i, _d, _i, _i, _i, _vp]),
+    "xlbh_record": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _d, _i, _i, _vp]),
 }
 
 _LIB = None

@@ -78,7 +78,7 @@ class _Checkpoint:
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, f0, force_field, stepper, bc_mask, missing_mask, omega, num_steps, every, ckpt_mode):
+    def forward(ctx, f0, force_field, stepper, bc_mask, missing_mask, omega, num_steps, every, ckpt_mode, overlap):
         col = stepper.collision
         if force_field is not None:
             if not isinstance(col, ForcedCollision) or col.force_field is None:
@@ -102,6 +102,7 @@ class _Rollout(torch.autograd.Function):
         ctx.force_shape = None if force_field is None else tuple(force_field.shape)
         ctx.force_dtype = None if force_field is None else force_field.dtype
         ctx.f0_dtype = f0.dtype
+        ctx.overlap = bool(overlap)
         return a
 
     @staticmethod
@@ -115,30 +116,70 @@ class _Rollout(torch.autograd.Function):
             grad_force = torch.zeros(ctx.force_shape, dtype=torch.float64, device=lam_a.device)
         n, k = ctx.num_steps, ctx.every
         shape = tuple(lam_a.shape)
-        states = torch.empty((max(1, min(k, n) - 1),) + shape, dtype=st.store_dtype, device=lam_a.device)
-        scratch = torch.empty(shape, dtype=st.store_dtype, device=lam_a.device) if any(
-            c.mode == "bf16" for c in ctx.ckpts.values()) else None
-        for s in sorted(ctx.ckpts.keys(), reverse=True):
+        starts = sorted(ctx.ckpts.keys(), reverse=True)
+        bf16 = any(c.mode == "bf16" for c in ctx.ckpts.values())
+        nbuf = 2 if ctx.overlap else 1
+        states = [torch.empty((max(1, min(k, n) - 1),) + shape, dtype=st.store_dtype, device=lam_a.device)
+                  for _ in range(nbuf)]
+        scratch = [torch.empty(shape, dtype=st.store_dtype, device=lam_a.device) if bf16 else None for _ in range(nbuf)]
+        main = torch.cuda.current_stream(lam_a.device)
+        side = torch.cuda.Stream(device=lam_a.device) if ctx.overlap else None
+
+        def record(idx):
+            """Restore the checkpoint of segment idx and record its states into buffer idx % nbuf."""
+            s = starts[idx]
+            b = idx % nbuf
+            f_s = ctx.ckpts[s].state(scratch[b], ctx.w)
+            st.record(f_s, states[b], bc_mask, missing_mask, ctx.omega, min(k, n - s) - 1, timestep=s)
+            return f_s
+
+        f_cur = record(0)
+        for idx, s in enumerate(starts):
             length = min(k, n - s)
-            f_s = ctx.ckpts[s].state(scratch, ctx.w)
-            out = st.adjoint_run(f_s, states, lam_a, lam_b, bc_mask, missing_mask, ctx.omega, length, timestep=s,
-                                 grad_force=grad_force)
+            f_next = None
+            if idx + 1 < len(starts) and side is not None:
+                # the buffers of segment idx + 1 were last read by the sweep enqueued on main
+                side.wait_stream(main)
+                with torch.cuda.stream(side):
+                    f_next = record(idx + 1)
+            out = st.adjoint_run(f_cur, states[idx % nbuf], lam_a, lam_b, bc_mask, missing_mask, ctx.omega, length,
+                                 timestep=s, grad_force=grad_force, recompute=False)
             if out is lam_b:
                 lam_a, lam_b = lam_b, lam_a
+            if idx + 1 < len(starts):
+                if side is not None:
+                    main.wait_stream(side)
+                else:
+                    f_next = record(idx + 1)
+            f_cur = f_next
+        if side is not None:
+            for t in states + [x for x in scratch if x is not None]:
+                t.record_stream(side)
         g_f0 = lam_a.to(ctx.f0_dtype)
         g_force = grad_force.to(ctx.force_dtype) if grad_force is not None else None
-        return g_f0, g_force, None, None, None, None, None, None, None
+        return g_f0, g_force, None, None, None, None, None, None, None, None
+
+
+# Grids up to this many cells leave the GPU partly idle in one step kernel, so the backward pass
+# records the next segment on a second stream while the current segment's adjoint runs.
+OVERLAP_MAX_CELLS = 1 << 22
 
 
 def rollout(stepper, f0, bc_mask, missing_mask, omega, num_steps, force_field=None, checkpoint_every=None,
-            checkpoint_dtype="store"):
+            checkpoint_dtype="store", overlap="auto"):
     """Differentiable ``num_steps``-step rollout; see the module docstring.
 
     checkpoint_dtype: "store" keeps checkpoints in the stepper's store dtype, "bf16" keeps f - w as
-    bfloat16 (half the memory; the recomputed trajectory then starts from the rounded state)."""
+    bfloat16 (half the memory; the recomputed trajectory then starts from the rounded state).
+    overlap: record segment i-1 on a second stream while the reverse sweep of segment i runs
+    (True / False / "auto" = grids of at most OVERLAP_MAX_CELLS cells); it doubles the recorded
+    state workspace."""
     if num_steps < 1:
         raise ValueError("num_steps must be >= 1")
     every = checkpoint_every or max(1, int(math.ceil(math.sqrt(num_steps))))
     if checkpoint_dtype not in ("store", "bf16"):
         raise ValueError("checkpoint_dtype must be 'store' or 'bf16'")
-    return _Rollout.apply(f0, force_field, stepper, bc_mask, missing_mask, omega, int(num_steps), int(every), checkpoint_dtype)
+    if overlap == "auto":
+        overlap = f0[0].numel() <= OVERLAP_MAX_CELLS
+    return _Rollout.apply(f0, force_field, stepper, bc_mask, missing_mask, omega, int(num_steps), int(every),
+                          checkpoint_dtype, bool(overlap))

@@ -116,6 +116,8 @@ struct AdjParams {
   const xlbh_bc_desc* table;
   const void* force_field;
   double* grad_force;
+  const int32_t* bc_cells;  // flat indices of the boundary cells, or nullptr (scan the grid)
+  int n_bc_cells;
   double fu[3];
   int force_mode;
   double omega;
@@ -250,15 +252,14 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
   }
 }
 
-// Kernel B: boundary cells add the contributions that do not follow the pull.
+// Boundary-cell contributions that do not follow the pull, for one boundary cell.  `part` selects
+// the work: -1 everything; 0 everything except the Zou-He / regularized Jacobian; 1 + k the
+// Jacobian column of input direction k only (so a cell's columns can run on separate threads).
 template <int LAT, int FMT, class C>
-__global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
+__device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cell, int x, int y, int z, int part) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
-  int x, y, z;
-  if (!sweep_cell(p.sw, x, y, z)) return;
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
-  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
   const int id = p.bc_mask[cell];
   if (id == 0 || id == 255) return;
   const xlbh_bc_desc* d = p.table + id;
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
       !heavy_kind(kind))
     return;
   const uint32_t mm = p.missing[cell];
+  if (part > 0 && (!heavy_kind(kind) || bit(mm, part - 1))) return;
   C fps[Q], h[Q], lam[Q];
   adj_prepare<LAT, FMT, C>(p, cell, x, y, z, kind, mm, d, fps, h, lam);
   C* li = static_cast<C*>(p.lam_in);
@@ -279,45 +281,48 @@ __global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
   for (int k = 0; k < Q; ++k) G[k] = lam[k];
   collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg);
 
-  if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+  if (part <= 0) {
+    if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
 #pragma unroll
-    for (int l = 0; l < Q; ++l)
-      if (bit(mm, l)) add(T.opp[l], cell, G[l]);  // h_l = f_own[opp l]
-  }
-  if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
-    // stash of slot opp(l): (1 - cs) f_own[opp l] + cs f^t_l(x - c_l - n)
-    int n[3];
-    bc_normal<LAT>(mm, n);
-    const C cs = C(0.57735026918962584);
-    const C* lo = static_cast<const C*>(p.lam_out);
+      for (int l = 0; l < Q; ++l)
+        if (bit(mm, l)) add(T.opp[l], cell, G[l]);  // h_l = f_own[opp l]
+    }
+    if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+      // stash of slot opp(l): (1 - cs) f_own[opp l] + cs f^t_l(x - c_l - n)
+      int n[3];
+      bc_normal<LAT>(mm, n);
+      const C cs = C(0.57735026918962584);
+      const C* lo = static_cast<const C*>(p.lam_out);
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
-      if (bit(mm, l)) {
-        const C ls = lo[(size_t)T.opp[l] * p.N + cell];
-        add(T.opp[l], cell, (C(1) - cs) * ls);
-        const int sx = wrap(x - (T.c[0][l] + n[0]), nx);
-        const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
-        const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
-        add(l, ((uint32_t)sx * ny + sy) * nz + sz, cs * ls);
+      for (int l = 0; l < Q; ++l) {
+        if (bit(mm, l)) {
+          const C ls = lo[(size_t)T.opp[l] * p.N + cell];
+          add(T.opp[l], cell, (C(1) - cs) * ls);
+          const int sx = wrap(x - (T.c[0][l] + n[0]), nx);
+          const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
+          const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
+          add(l, ((uint32_t)sx * ny + sy) * nz + sz, cs * ls);
+        }
+      }
+    }
+    if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+      for (int k = 0; k < Q; ++k) add(k, cell, G[k]);
+    }
+    if (heavy_kind(kind) || kind == XLBH_BC_DO_NOTHING) {
+      if (has_force && p.grad_force) {
+#pragma unroll
+        for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
       }
     }
   }
-  if (kind == XLBH_BC_DO_NOTHING) {
-#pragma unroll
-    for (int k = 0; k < Q; ++k) add(k, cell, G[k]);
-  }
-  if (heavy_kind(kind) || kind == XLBH_BC_DO_NOTHING) {
-    if (has_force && p.grad_force) {
-#pragma unroll
-      for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
-    }
-  }
-  if (heavy_kind(kind)) {
+  if (heavy_kind(kind) && part != 0) {
     // G_ps = (d h / d f_ps)^T G_h: one dual pass per input direction k.
     const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
     const bool vel = (kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY);
     const bool reg = kind >= XLBH_BC_REGULARIZED_VELOCITY;
-    for (int k = 0; k < Q; ++k) {
+    const int k0 = part > 0 ? part - 1 : 0, k1 = part > 0 ? part : Q;
+    for (int k = k0; k < k1; ++k) {
       if (bit(mm, k)) continue;  // overwritten by the BC: no dependence
       Dual<C> hd[Q];
 #pragma unroll
@@ -332,11 +337,45 @@ __global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
   }
 }
 
+// Kernel B over the whole grid (no boundary-cell list given).
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const uint32_t cell = ((uint32_t)x * p.sw.ny + y) * p.sw.nz + z;
+  adjoint_bc_cell<LAT, FMT, C>(p, cell, x, y, z, -1);
+}
+
+// Kernel B over a list of boundary cells: Q + 1 threads per cell (part 0 and the Q Jacobian
+// columns).
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_adjoint_bc_list(AdjParams p) {
+  constexpr int Q = make_tables(LAT).q;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)p.n_bc_cells * (Q + 1)) return;
+  const int i = (int)(t / (Q + 1)), part = (int)(t % (Q + 1));
+  const int32_t c = p.bc_cells[i];
+  if (c < 0 || (uint32_t)c >= p.N) return;
+  const uint32_t cell = (uint32_t)c;
+  const int nz = p.sw.nz, ny = p.sw.ny;
+  const int z = (int)(cell % (uint32_t)nz);
+  const int y = (int)((cell / (uint32_t)nz) % (uint32_t)ny);
+  const int x = (int)(cell / ((uint32_t)nz * (uint32_t)ny));
+  adjoint_bc_cell<LAT, FMT, C>(p, cell, x, y, z, part);
+}
+
 template <int LAT, int FMT, class C>
 int launch_adjoint(const AdjParams& p, hipStream_t s) {
   hipLaunchKernelGGL((k_adjoint_push<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
   XLBH_LAUNCH_CHECK();
-  if (p.bc_mask) {
+  if (p.bc_mask && p.bc_cells) {
+    if (p.n_bc_cells > 0) {
+      constexpr int Q = make_tables(LAT).q;
+      const int64_t threads = (int64_t)p.n_bc_cells * (Q + 1);
+      hipLaunchKernelGGL((k_adjoint_bc_list<LAT, FMT, C>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, p);
+      XLBH_LAUNCH_CHECK();
+    }
+  } else if (p.bc_mask) {
     hipLaunchKernelGGL((k_adjoint_bc<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
     XLBH_LAUNCH_CHECK();
   }
@@ -390,7 +429,7 @@ static int adjoint_validate(int lattice, int force_mode, const double* force_uni
 
 static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam_in, const uint8_t* bc_mask,
                                 const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
-                                int force_mode, const double* force_uniform, const void* force_field,
+                                const int32_t* bc_cells, int n_bc_cells, int force_mode, const double* force_uniform, const void* force_field,
                                 double* grad_force, int nx, int ny, int nz, double omega) {
   AdjParams p;
   p.f_in = f_in;
@@ -400,6 +439,8 @@ static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam
   p.missing = missing_bits;
   p.aux = aux;
   p.table = bc_table;
+  p.bc_cells = bc_cells;
+  p.n_bc_cells = n_bc_cells;
   p.force_field = force_field;
   p.grad_force = grad_force;
   p.force_mode = force_mode;
@@ -412,14 +453,16 @@ static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam
 
 extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void* f_ckpt, void* states, void* lam_a,
                                 void* lam_b, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
-                                const xlbh_bc_desc* bc_table, int force_mode, const double* force_uniform,
+                                const xlbh_bc_desc* bc_table, const int32_t* bc_cells, int n_bc_cells,
+                                int force_mode, const double* force_uniform,
                                 const void* force_field, double* grad_force, int nx, int ny, int nz, double omega,
-                                int timestep0, int n_steps, void* stream) {
+                                int timestep0, int n_steps, int recompute, void* stream) {
   clear_error();
   if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
                                ny, nz))
     return e;
   XLBH_REQUIRE(n_steps >= 1, "n_steps must be >= 1");
+  XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
   XLBH_REQUIRE(f_ckpt && lam_a && lam_b, "adjoint_run needs f_ckpt, lam_a and lam_b");
   XLBH_REQUIRE(lam_a != lam_b, "lam_a must not alias lam_b");
   XLBH_REQUIRE(n_steps == 1 || states, "adjoint_run needs a state workspace for n_steps > 1");
@@ -428,8 +471,8 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
   auto state = [&](int j) -> const void* {
     return j == 0 ? f_ckpt : static_cast<const void*>(static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes);
   };
-  // recompute f^{t0+1} .. f^{t0+n-1}
-  for (int j = 1; j < n_steps; ++j) {
+  // recompute f^{t0+1} .. f^{t0+n-1} (unless the caller recorded them, e.g. with xlbh_record)
+  for (int j = 1; recompute && j < n_steps; ++j) {
     if (int e = xlbh_step(lattice, XLBH_BGK, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
                           missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega,
                           timestep0 + j - 1, stream))
@@ -438,7 +481,8 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
   void* cur = lam_a;
   void* nxt = lam_b;
   for (int j = n_steps - 1; j >= 0; --j) {
-    const AdjParams p = adjoint_params(state(j), cur, nxt, bc_mask, missing_bits, aux, bc_table, force_mode,
+    const AdjParams p = adjoint_params(state(j), cur, nxt, bc_mask, missing_bits, aux, bc_table, bc_cells,
+                                       n_bc_cells, force_mode,
                                        force_uniform, force_field, grad_force, nx, ny, nz, omega);
     if (int e = adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream)) return e;
     void* t = cur;
@@ -450,8 +494,9 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
 
 extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void* f_in, const void* lam_out,
                                  void* lam_in, const uint8_t* bc_mask, const uint32_t* missing_bits,
-                                 const void* aux, const xlbh_bc_desc* bc_table, int force_mode,
-                                 const double* force_uniform, const void* force_field, double* grad_force, int nx,
+                                 const void* aux, const xlbh_bc_desc* bc_table, const int32_t* bc_cells,
+                                 int n_bc_cells, int force_mode, const double* force_uniform,
+                                 const void* force_field, double* grad_force, int nx,
                                  int ny, int nz, double omega, void* stream) {
   clear_error();
   if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
@@ -459,7 +504,9 @@ extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void
     return e;
   XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
   XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
-  const AdjParams p = adjoint_params(f_in, lam_out, lam_in, bc_mask, missing_bits, aux, bc_table, force_mode,
+  XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
+  const AdjParams p = adjoint_params(f_in, lam_out, lam_in, bc_mask, missing_bits, aux, bc_table, bc_cells,
+                                     n_bc_cells, force_mode,
                                      force_uniform, force_field, grad_force, nx, ny, nz, omega);
   return adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream);
 }

@@ -91,4 +91,27 @@ int xlbh_run(int lattice, int collision, int store, int compute, void* f_a, void
   return 0;
 }
 
+int xlbh_record(int lattice, int collision, int store, int compute, const void* f_start, void* states,
+                const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
+                int force_mode, const double* force_uniform, const void* force_field, int nx, int ny, int nz,
+                double omega, int timestep0, int n_states, void* stream) {
+  clear_error();
+  XLBH_REQUIRE(n_states >= 0, "n_states must be >= 0");
+  if (n_states == 0) return 0;
+  XLBH_REQUIRE(f_start && states, "record needs f_start and states");
+  XLBH_REQUIRE(valid_store(store), "unknown store format");
+  if (int e = validate_step(lattice, collision, store, compute, f_start, states, bc_mask, missing_bits, bc_table,
+                            force_mode, force_uniform, force_field, nx, ny, nz))
+    return e;
+  const size_t field_bytes = (size_t)lattice_q(lattice) * (size_t)nx * ny * nz * store_bytes(store);
+  StepCall c{collision, store, compute, f_start, states, bc_mask, missing_bits, aux, bc_table,
+             force_mode, force_uniform, force_field, nx, ny, nz, omega, (hipStream_t)stream};
+  for (int j = 0; j < n_states; ++j) {
+    c.f_in = j == 0 ? f_start : static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes;
+    c.f_out = static_cast<char*>(states) + (size_t)j * field_bytes;
+    if (int e = dispatch_step(lattice, c)) return e;
+  }
+  return 0;
+}
+
 }  // extern "C"

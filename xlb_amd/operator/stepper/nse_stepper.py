@@ -24,7 +24,7 @@ from xlb_amd.helper.check_boundary_overlaps import check_bc_overlaps
 from xlb_amd.helper.nse_solver import create_nse_fields
 from xlb_amd.operator.boundary_condition.boundary_condition import ImplementationStep
 from xlb_amd.operator.boundary_masker import IndicesBoundaryMasker
-from xlb_amd.operator.boundary_masker.packing import packed_missing
+from xlb_amd.operator.boundary_masker.packing import cells_with_id, packed_missing
 from xlb_amd.operator.collision import BGK, KBC, ForcedCollision, SmagorinskyLESBGK
 from xlb_amd.operator.equilibrium import QuadraticEquilibrium
 from xlb_amd.operator.macroscopic import Macroscopic
@@ -83,6 +83,8 @@ class IncompressibleNavierStokesStepper(Stepper):
         self._aux = None
         self._has_bc_cells = True
         self._has_solid = False
+        self._bc_cells = None
+        self._mask_ref = None
 
     # ------------------------------------------------------------------ fields
     def prepare_fields(self, initializer=None):
@@ -153,6 +155,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         self._table = self._build_table(bc_mask.device)
         self._has_bc_cells = bool(bc_mask.any().item())
         self._has_solid = self._has_bc_cells and bool((bc_mask == 255).any().item())
+        self._bc_cells = None
         nx, ny, nz = field_extents(bc_mask)
         aux_bcs = [bc for bc in self.boundary_conditions if bc.needs_aux_init]
         if aux_bcs and (self._aux is None or self._aux.numel() != nx * ny * nz or self._state_key is not None):
@@ -160,6 +163,7 @@ class IncompressibleNavierStokesStepper(Stepper):
             self._aux = torch.zeros(nx * ny * nz, dtype=self.compute_dtype, device=bc_mask.device)
             for bc in aux_bcs:
                 bc.aux_data_init(None, None, bc_mask, missing_mask, aux=self._aux)
+        self._mask_ref = bc_mask
         self._state_key = self._mask_key(bc_mask, missing_mask)
 
     def _force_args(self):
@@ -212,6 +216,26 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.mark_written(f_1)
         return f_0, f_1
 
+    _ADJOINT_BC_KINDS = (
+        _lib.BC_HALFWAY_BB, _lib.BC_EXTRAPOLATION_OUTFLOW, _lib.BC_DO_NOTHING, _lib.BC_ZOUHE_VELOCITY,
+        _lib.BC_ZOUHE_PRESSURE, _lib.BC_REGULARIZED_VELOCITY, _lib.BC_REGULARIZED_PRESSURE,
+    )
+
+    def _adjoint_cells(self):
+        """(pointer, count) of the cells the adjoint's boundary pass runs over: every cell whose BC
+        kind has contributions outside the pull (bounce-back, extrapolation, do-nothing, Zou-He /
+        regularized); fullway bounce-back and equilibrium cells have none."""
+        if not self._has_bc_cells:
+            return None, 0
+        if self._bc_cells is None:
+            ids = [int(bc.id) for bc in self.boundary_conditions if bc.kind in self._ADJOINT_BC_KINDS]
+            parts = [cells_with_id(self._mask_ref, i) for i in ids]
+            parts = [p for p in parts if p.numel() > 0]
+            self._bc_cells = torch.cat(parts) if parts else torch.zeros(0, dtype=torch.int32, device=self._mask_ref.device)
+        if self._bc_cells.numel() == 0:
+            return None, 0
+        return _lib.ptr(self._bc_cells), int(self._bc_cells.numel())
+
     def adjoint_step(self, f_t, lam_out, lam_in, bc_mask, missing_mask, omega, grad_force=None):
         """Reverse mode of one step (xlbh_adjoint_step): given the state f_t a forward step read and
         lam_out = dL/df_{t+1}, write lam_in = dL/df_t and add dL/dg to ``grad_force`` ((d, nx, ny, nz)
@@ -231,7 +255,8 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.check(
             _lib.lib().xlbh_adjoint_step(
                 self.lattice_id, self.precision_policy.store_format, self.compute_id,
-                _lib.ptr(f_t), _lib.ptr(lam_out), _lib.ptr(lam_in), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
+                _lib.ptr(f_t), _lib.ptr(lam_out), _lib.ptr(lam_in), *masks, *self._adjoint_cells(), force_mode, fu_ptr,
+                _lib.ptr(force_field),
                 _lib.ptr(grad_force), nx, ny, nz, float(omega), _lib.stream_of(f_t),
             ),
             "IncompressibleNavierStokesStepper.adjoint_step",
@@ -239,12 +264,34 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.mark_written(lam_in, grad_force)
         return lam_in
 
+    def record(self, f_start, states, bc_mask, missing_mask, omega, num_states, timestep=0):
+        """``num_states`` steps from ``f_start`` keeping every state (xlbh_record): states[j] is the
+        output of step ``timestep + j``.  ``states`` is a contiguous (>= num_states, q, ...) tensor
+        in the store dtype."""
+        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_start, None, bc_mask, missing_mask)
+        if num_states < 0:
+            raise ValueError("num_states must be >= 0")
+        if num_states > 0 and (states is None or states.dtype != self.store_dtype or not states.is_contiguous()
+                               or states.shape[0] < num_states or tuple(states.shape[1:]) != tuple(f_start.shape)):
+            raise ValueError("states must be a contiguous (num_states, q, nx, ny, nz) store-dtype tensor")
+        _lib.check(
+            _lib.lib().xlbh_record(
+                self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
+                _lib.ptr(f_start), _lib.ptr(states), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
+                nx, ny, nz, float(omega), int(timestep), int(num_states), _lib.stream_of(f_start),
+            ),
+            "IncompressibleNavierStokesStepper.record",
+        )
+        _lib.mark_written(states)
+        return states
+
     def adjoint_run(self, f_ckpt, states, lam_a, lam_b, bc_mask, missing_mask, omega, num_steps, timestep=0,
-                    grad_force=None):
+                    grad_force=None, recompute=True):
         """Reverse sweep over one checkpoint segment below the ABI (xlbh_adjoint_run): recompute the
         ``num_steps - 1`` states after ``f_ckpt`` into ``states`` ((num_steps - 1, q, ...) in the
         store dtype), then apply the adjoint steps in reverse starting from lam_a = dL/df at the
-        segment end.  Returns the buffer (lam_a or lam_b) that holds dL/df_ckpt.  BGK only."""
+        segment end.  Returns the buffer (lam_a or lam_b) that holds dL/df_ckpt.  With
+        ``recompute=False`` the states are taken as recorded by ``record``.  BGK only."""
         if self.collision_type != "BGK":
             raise NotImplementedError("the adjoint step is implemented for BGK collision")
         if self._has_solid:
@@ -266,9 +313,9 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.check(
             _lib.lib().xlbh_adjoint_run(
                 self.lattice_id, self.precision_policy.store_format, self.compute_id,
-                _lib.ptr(f_ckpt), _lib.ptr(states), _lib.ptr(lam_a), _lib.ptr(lam_b), *masks, force_mode, fu_ptr,
-                _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, float(omega), int(timestep), int(num_steps),
-                _lib.stream_of(f_ckpt),
+                _lib.ptr(f_ckpt), _lib.ptr(states), _lib.ptr(lam_a), _lib.ptr(lam_b), *masks, *self._adjoint_cells(),
+                force_mode, fu_ptr, _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, float(omega),
+                int(timestep), int(num_steps), int(bool(recompute)), _lib.stream_of(f_ckpt),
             ),
             "IncompressibleNavierStokesStepper.adjoint_run",
         )
