@@ -127,7 +127,7 @@ int xlbh_run(int lattice, int collision, int store, int compute,
 
 /* n_states steps from f_start, keeping every state: state j (the output of step t0 + j) is
  * written to states + j * (Q N S_store) bytes, for j = 0 .. n_states - 1.  The forward pass of a
- * checkpointed reverse sweep (xlbh_adjoint_run with recompute = 0); no counterpart in the
+ * checkpointed reverse sweep (xlbh_adjoint_run without XLBH_ADJ_RECOMPUTE); no counterpart in the
  * reference, whose tape keeps the segment's states itself. */
 int xlbh_record(int lattice, int collision, int store, int compute, const void* f_start, void* states,
                 const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
@@ -257,9 +257,16 @@ int xlbh_adjoint_step(int lattice, int store, int compute,
  *            recomputed f^{t0+1} .. f^{t0+n_steps-1}.  May be NULL when n_steps == 1.
  *   lam_a  : dL/df^{t0+n_steps} on entry.  lam_b: a second (Q, N) compute-type buffer.
  * The result dL/df^{t0} is left in lam_a when n_steps is even and in lam_b when it is odd; the
- * other buffer is overwritten.  grad_force and bc_cells as in xlbh_adjoint_step.  With
- * recompute = 0 the states are taken as already recorded (xlbh_record), so the recompute of one
- * segment can run on another stream while the reverse sweep of the next runs on this one. */
+ * other buffer is overwritten.  grad_force and bc_cells as in xlbh_adjoint_step.
+ * flags: XLBH_ADJ_RECOMPUTE recomputes the states first; without it they are taken as already
+ *   recorded (xlbh_record), so the recording of one segment can run on another stream while the
+ *   reverse sweep of the next runs on this one.
+ * workspace: NULL, or xlbh_adjoint_workspace_bytes() bytes that select the fused reverse step
+ *   (one kernel per step; boundary contributions are carried to the next reverse step in the
+ *   workspace instead of being added by a second kernel).  A sweep over several segments passes
+ *   the same workspace to every call, with XLBH_ADJ_BEGIN on its first call (clears it) and
+ *   XLBH_ADJ_END on its last (adds the carried contributions into the result).  Calls that share
+ *   one workspace must not run concurrently. */
 int xlbh_adjoint_run(int lattice, int store, int compute,
                      const void* f_ckpt, void* states, void* lam_a, void* lam_b,
                      const uint8_t* bc_mask, const uint32_t* missing_bits,
@@ -267,7 +274,15 @@ int xlbh_adjoint_run(int lattice, int store, int compute,
                      const int32_t* bc_cells, int n_bc_cells,
                      int force_mode, const double* force_uniform, const void* force_field,
                      double* grad_force, int nx, int ny, int nz, double omega, int timestep0,
-                     int n_steps, int recompute, void* stream);
+                     int n_steps, void* workspace, int flags, void* stream);
+
+#define XLBH_ADJ_RECOMPUTE 1
+#define XLBH_ADJ_BEGIN 2
+#define XLBH_ADJ_END 4
+
+/* Bytes of the fused reverse sweep's workspace: three (Q, N) fields of the compute type and N
+ * flag bytes, rounded up to 256 (-1 for invalid arguments). */
+int64_t xlbh_adjoint_workspace_bytes(int lattice, int compute, int nx, int ny, int nz);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,7 @@ BC_REGULARIZED_VELOCITY = 7
 BC_REGULARIZED_PRESSURE = 8
 BC_EXTRAPOLATION_OUTFLOW = 9
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
+ADJ_RECOMPUTE, ADJ_BEGIN, ADJ_END = 1, 2, 4
 
 ABI_VERSION = 1
 
@@ -74,7 +75,8 @@ SIGNATURES = {
     "xlbh_adjoint_step": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i, _i,
                                _d, _vp]),
     "xlbh_adjoint_run": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
-                              _i, _d, _i, _i, _i, _vp]),
+                              _i, _d, _i, _i, _vp, _i, _vp]),
+    "xlbh_adjoint_workspace_bytes": (_i64, [_i, _i, _i, _i, _i]),
     "xlbh_record": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _d, _i, _i, _vp]),
 }
 

@@ -124,6 +124,8 @@ class _Rollout(torch.autograd.Function):
         scratch = [torch.empty(shape, dtype=st.store_dtype, device=lam_a.device) if bf16 else None for _ in range(nbuf)]
         main = torch.cuda.current_stream(lam_a.device)
         side = torch.cuda.Stream(device=lam_a.device) if ctx.overlap else None
+        # small grids: the fused one-kernel reverse step (launch-bound regime)
+        workspace = st.adjoint_workspace(bc_mask) if ctx.overlap else None
 
         def record(idx):
             """Restore the checkpoint of segment idx and record its states into buffer idx % nbuf."""
@@ -143,7 +145,8 @@ class _Rollout(torch.autograd.Function):
                 with torch.cuda.stream(side):
                     f_next = record(idx + 1)
             out = st.adjoint_run(f_cur, states[idx % nbuf], lam_a, lam_b, bc_mask, missing_mask, ctx.omega, length,
-                                 timestep=s, grad_force=grad_force, recompute=False)
+                                 timestep=s, grad_force=grad_force, recompute=False, workspace=workspace,
+                                 begin=(idx == 0), end=(idx == len(starts) - 1))
             if out is lam_b:
                 lam_a, lam_b = lam_b, lam_a
             if idx + 1 < len(starts):
@@ -160,8 +163,9 @@ class _Rollout(torch.autograd.Function):
         return g_f0, g_force, None, None, None, None, None, None, None, None
 
 
-# Grids up to this many cells leave the GPU partly idle in one step kernel, so the backward pass
-# records the next segment on a second stream while the current segment's adjoint runs.
+# Grids up to this many cells leave the GPU partly idle in one step kernel: the backward pass then
+# records the next segment on a second stream while the current segment's adjoint runs, and uses
+# the fused one-kernel reverse step.
 OVERLAP_MAX_CELLS = 1 << 22
 
 

@@ -106,6 +106,87 @@ __device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, boo
   }
 }
 
+// Reverse mode of zouhe_cell (physics.hpp) in one pass: given G = dL/dh for h = zouhe_cell(f),
+// returns Gf = dL/df (zero for the missing directions, which the functional overwrites).
+// Forward: fsum = sum_l a_l f_l (a = 2 opposite-of-missing, 1 middle, 0 missing); velocity:
+// rho = fsum / (1 + u.n) with u prescribed; pressure: u = (-1 + fsum / rho) n with rho prescribed;
+// missing l (in order): f_l = f_opp(l) + feq_l - feq_opp(l); regularized: f_l = feq_l +
+// 4.5 w_l sum_t qi_lt Pi_t with Pi_t = sum_m cc_mt (f_m - feq_m).  The in-place updates are undone
+// in reverse order; the equilibria's dependence on (rho, u) uses the moments A, B of eq_moments.
+template <int LAT, class C>
+__device__ __forceinline__ void zouhe_cell_adjoint(const C* f, uint32_t mm, C pv, bool velocity, bool regularize,
+                                                   const C* G, C* Gf) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int ni[3];
+  bc_normal<LAT>(mm, ni);
+  const C n[3] = {C(ni[0]), C(ni[1]), C(ni[2])};
+  const C fsum = bc_fsum<LAT>(f, mm);
+  C rho, u[3];
+  C un = C(0);
+  if (velocity) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = -pv * n[d];
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) un += u[d] * n[d];
+    rho = fsum / (C(1) + un);
+  } else {
+    rho = pv;
+    un = -C(1) + fsum / rho;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = un * n[d];
+  }
+  C Gfeq[Q];
+  if (regularize) {
+    C P[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+#pragma unroll
+      for (int t = 0; t < T.nt; ++t) P[t] += G[l] * C(4.5) * C(T.w[l]) * C(T.qi[l][t]);
+#pragma unroll
+    for (int m = 0; m < Q; ++m) {
+      C g = C(0);
+#pragma unroll
+      for (int t = 0; t < T.nt; ++t) g += C(T.cc[m][t]) * P[t];
+      Gf[m] = g;
+      Gfeq[m] = G[m] - g;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < Q; ++m) {
+      Gf[m] = G[m];
+      Gfeq[m] = C(0);
+    }
+  }
+#pragma unroll
+  for (int l = Q - 1; l >= 0; --l) {
+    if (bit(mm, l)) {
+      const int o = T.opp[l];
+      const C g = Gf[l];
+      Gf[o] += g;
+      Gfeq[l] += g;
+      Gfeq[o] -= g;
+      Gf[l] = C(0);
+    }
+  }
+  C A, B[3];
+  eq_moments<LAT>(Gfeq, u, u, A, B);
+  C gfsum;
+  if (velocity) {
+    gfsum = A / (C(1) + un);
+  } else {
+    C gun = C(0);
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) gun += rho * B[d] * n[d];
+    gfsum = gun / rho;
+  }
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    if (bit(mm, T.opp[l])) Gf[l] += C(2) * gfsum;
+    else if (!bit(mm, l)) Gf[l] += gfsum;
+  }
+}
+
 struct AdjParams {
   const void* f_in;
   const void* lam_out;
@@ -118,6 +199,11 @@ struct AdjParams {
   double* grad_force;
   const int32_t* bc_cells;  // flat indices of the boundary cells, or nullptr (scan the grid)
   int n_bc_cells;
+  // Fused reverse step (workspace given): per-cell deltas carried to the next reverse step.
+  void* d_own;           // (Q, N): contributions to dL/df_in at the producing cell itself
+  void* d_nb_in;         // (Q, N): neighbour contributions produced by the previous reverse step
+  void* d_nb_out;        // (Q, N): neighbour contributions produced by this reverse step
+  const uint8_t* nb_flag;  // (N): 1 where d_nb entries can arrive
   double fu[3];
   int force_mode;
   double omega;
@@ -126,10 +212,12 @@ struct AdjParams {
 };
 
 // The streaming-step view of one cell: post-BC populations h, the adjoint of the output with the
-// slots overwritten after collision removed, and the BC kind.  Shared by kernels A and B.
+// slots overwritten after collision removed, and the BC kind.  Shared by every adjoint kernel.
+// `lamf`, when given, is the cell's dL/df_out (otherwise it is read from lam_out).
 template <int LAT, int FMT, class C>
 __device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, int x, int y, int z, int kind,
-                                            uint32_t mm, const xlbh_bc_desc* d, C* fps, C* h, C* lam) {
+                                            uint32_t mm, const xlbh_bc_desc* d, C* fps, C* h, C* lam,
+                                            const C* lamf = nullptr) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   using R = typename Fmt<FMT>::raw;
@@ -140,7 +228,7 @@ __device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, i
   for (int l = 0; l < Q; ++l) {
     const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
     fps[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
-    lam[l] = lo[(size_t)l * p.N + cell];
+    lam[l] = lamf ? lamf[l] : lo[(size_t)l * p.N + cell];
     h[l] = fps[l];
   }
   if (kind == XLBH_BC_EQUILIBRIUM) {
@@ -364,6 +452,221 @@ __global__ __launch_bounds__(256) void k_adjoint_bc_list(AdjParams p) {
   adjoint_bc_cell<LAT, FMT, C>(p, cell, x, y, z, part);
 }
 
+// ------------------------------------------------------------------------------------------
+// Fused reverse step: one kernel per step.  The contributions that kernel B adds with atomics
+// land either in slots the cell pushes itself (the Zou-He / regularized Jacobian: added before
+// the push), or in slots (k, x) at the producing cell x itself (bounce-back, do-nothing and the
+// extrapolation stash's own part), which the next reverse step reads at cell x only: they are
+// kept per cell in d_own and read back by the same thread one step later.  The extrapolation
+// neighbour part lands on cell x - c_l - n; it goes to d_nb (double-buffered by step parity) and
+// is read and cleared by the flagged consumer cell in the next reverse step.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool own_kind(int kind) {
+  return kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW || kind == XLBH_BC_DO_NOTHING;
+}
+
+template <int LAT, int FMT, class C>
+__global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+  const int id = p.bc_mask ? p.bc_mask[cell] : 0;
+  C* li = static_cast<C*>(p.lam_in);
+  if (id == 255) {
+    // solid cells are not updated by the forward step: nothing flows back through their pulls
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+      li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = C(0);
+    }
+    return;
+  }
+  int kind = XLBH_BC_NONE;
+  uint32_t mm = 0;
+  const xlbh_bc_desc* d = nullptr;
+  if (id != 0) {
+    d = p.table + id;
+    kind = d->kind;
+    mm = p.missing[cell];
+  }
+  // full dL/df_out at this cell: the pushes of the previous reverse step plus its deltas
+  const C* lo = static_cast<const C*>(p.lam_out);
+  C* dob = static_cast<C*>(p.d_own);
+  C lamf[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) lamf[l] = lo[(size_t)l * p.N + cell];
+  if (own_kind(kind)) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) lamf[l] += dob[(size_t)l * p.N + cell];
+  }
+  if (p.nb_flag[cell]) {
+    C* dni = static_cast<C*>(p.d_nb_in);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      lamf[l] += dni[(size_t)l * p.N + cell];
+      dni[(size_t)l * p.N + cell] = C(0);
+    }
+  }
+  C fps[Q], h[Q], lam[Q];
+  adj_prepare<LAT, FMT, C>(p, cell, x, y, z, kind, mm, d, fps, h, lam, lamf);
+  C G[Q], dn[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) dn[k] = C(0);
+  if (kind == XLBH_BC_FULLWAY_BB) {
+#pragma unroll
+    for (int k = 0; k < Q; ++k) G[k] = lam[T.opp[k]];
+  } else {
+    C F[3], gg[3], Gh[Q];
+    cell_force<LAT>(p, cell, F);
+    const bool has_force = p.force_mode != XLBH_FORCE_NONE;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) Gh[k] = lam[k];
+    collide_adjoint<LAT>(h, Gh, (C)p.omega, has_force, F, gg);
+    if (has_force && p.grad_force) {
+#pragma unroll
+      for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
+    }
+    if (kind == XLBH_BC_EQUILIBRIUM) {
+#pragma unroll
+      for (int k = 0; k < Q; ++k) G[k] = C(0);
+    } else if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        G[k] = C(0);
+        dn[k] = Gh[k];  // h = f_own
+      }
+    } else if (heavy_kind(kind)) {
+      // G_ps = (d h / d f_ps)^T G_h in one reverse pass
+      const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
+      zouhe_cell_adjoint<LAT>(fps, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
+                              kind >= XLBH_BC_REGULARIZED_VELOCITY, Gh, G);
+    } else {
+#pragma unroll
+      for (int k = 0; k < Q; ++k) G[k] = Gh[k];
+      if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+          if (bit(mm, l)) {
+            G[l] = C(0);             // not pulled
+            dn[T.opp[l]] += Gh[l];   // h_l = f_own[opp l]
+          }
+      }
+      if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+        // stash of slot opp(l): (1 - cs) f_own[opp l] + cs f^t_l(x - c_l - n)
+        int n[3];
+        bc_normal<LAT>(mm, n);
+        const C cs = C(0.57735026918962584);
+        C* dno = static_cast<C*>(p.d_nb_out);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+          if (bit(mm, l)) {
+            const C ls = lamf[T.opp[l]];
+            dn[T.opp[l]] += (C(1) - cs) * ls;
+            const int sx = wrap(x - (T.c[0][l] + n[0]), nx);
+            const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
+            const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
+            atomicAdd(dno + (size_t)l * p.N + ((uint32_t)sx * ny + sy) * nz + sz, cs * ls);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+    li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = G[k];
+  }
+  if (own_kind(kind)) {
+#pragma unroll
+    for (int k = 0; k < Q; ++k) dob[(size_t)k * p.N + cell] = dn[k];
+  }
+}
+
+// Marks the cells that receive extrapolation neighbour contributions (nb_flag must be zero).
+template <int LAT>
+__global__ __launch_bounds__(256) void k_adjoint_nb_flags(AdjParams p, uint8_t* flag) {
+  constexpr LatTables T = make_tables(LAT);
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+  const int id = p.bc_mask[cell];
+  if (id == 0 || id == 255 || p.table[id].kind != XLBH_BC_EXTRAPOLATION_OUTFLOW) return;
+  const uint32_t mm = p.missing[cell];
+  int n[3];
+  bc_normal<LAT>(mm, n);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    if (bit(mm, l)) {
+      const int sx = wrap(x - (T.c[0][l] + n[0]), nx);
+      const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
+      const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
+      flag[((uint32_t)sx * ny + sy) * nz + sz] = 1;
+    }
+  }
+}
+
+// dL/df of the sweep's last step: the pushed values plus the deltas it left.
+template <int LAT, class C>
+__global__ __launch_bounds__(256) void k_adjoint_flush(AdjParams p, C* lam) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  int x, y, z;
+  if (!sweep_cell(p.sw, x, y, z)) return;
+  const uint32_t cell = ((uint32_t)x * p.sw.ny + y) * p.sw.nz + z;
+  const int id = p.bc_mask ? p.bc_mask[cell] : 0;
+  const bool own = id != 0 && id != 255 && own_kind(p.table[id].kind);
+  const bool nb = p.nb_flag[cell] != 0;
+  if (!own && !nb) return;
+  const C* dob = static_cast<const C*>(p.d_own);
+  const C* dno = static_cast<const C*>(p.d_nb_out);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    C v = lam[(size_t)l * p.N + cell];
+    if (own) v += dob[(size_t)l * p.N + cell];
+    if (nb) v += dno[(size_t)l * p.N + cell];
+    lam[(size_t)l * p.N + cell] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* p, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256u;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+enum AdjMode { ADJ_SPLIT = 0, ADJ_FUSED = 1, ADJ_BEGIN = 2, ADJ_FLUSH = 3 };
+
+template <int LAT, int FMT, class C>
+int launch_adjoint(const AdjParams& p, hipStream_t s);
+
+// The fused path's kernels.  `ws_words` / `flag` / `lam` are used by BEGIN / FLUSH.
+template <int LAT, int FMT, class C>
+int launch_adjoint_mode(const AdjParams& p, hipStream_t s, int mode, uint64_t ws_words, uint8_t* flag, void* lam) {
+  if (mode == ADJ_SPLIT) return launch_adjoint<LAT, FMT, C>(p, s);
+  if (mode == ADJ_FUSED) {
+    hipLaunchKernelGGL((k_adjoint_fused<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
+    XLBH_LAUNCH_CHECK();
+    return 0;
+  }
+  if (mode == ADJ_BEGIN) {
+    const uint64_t blocks = std::min<uint64_t>((ws_words + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<uint32_t*>(p.d_own),
+                       ws_words);
+    XLBH_LAUNCH_CHECK();
+    if (p.bc_mask) {
+      hipLaunchKernelGGL((k_adjoint_nb_flags<LAT>), sweep_grid(p.sw), dim3(256), 0, s, p, flag);
+      XLBH_LAUNCH_CHECK();
+    }
+    return 0;
+  }
+  hipLaunchKernelGGL((k_adjoint_flush<LAT, C>), sweep_grid(p.sw), dim3(256), 0, s, p, static_cast<C*>(lam));
+  XLBH_LAUNCH_CHECK();
+  return 0;
+}
+
 template <int LAT, int FMT, class C>
 int launch_adjoint(const AdjParams& p, hipStream_t s) {
   hipLaunchKernelGGL((k_adjoint_push<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
@@ -386,21 +689,25 @@ int launch_adjoint(const AdjParams& p, hipStream_t s) {
 
 using namespace xlbh;
 
-static int adjoint_dispatch(int lattice, int store, int compute, const AdjParams& p, hipStream_t s) {
+static int adjoint_dispatch(int lattice, int store, int compute, const AdjParams& p, hipStream_t s,
+                            int mode = ADJ_SPLIT, uint64_t ws_words = 0, uint8_t* flag = nullptr,
+                            void* lam = nullptr) {
   auto go = [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
     if (compute == XLBH_COMPUTE_F32) {
       switch (store) {
-        case XLBH_F32: return launch_adjoint<LAT, XLBH_F32, float>(p, s);
-        case XLBH_F16: return launch_adjoint<LAT, XLBH_F16, float>(p, s);
-        case XLBH_BF16: return launch_adjoint<LAT, XLBH_BF16, float>(p, s);
-        case XLBH_F16_SHIFTED: return launch_adjoint<LAT, XLBH_F16_SHIFTED, float>(p, s);
-        case XLBH_BF16_SHIFTED: return launch_adjoint<LAT, XLBH_BF16_SHIFTED, float>(p, s);
+        case XLBH_F32: return launch_adjoint_mode<LAT, XLBH_F32, float>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F16: return launch_adjoint_mode<LAT, XLBH_F16, float>(p, s, mode, ws_words, flag, lam);
+        case XLBH_BF16: return launch_adjoint_mode<LAT, XLBH_BF16, float>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F16_SHIFTED:
+          return launch_adjoint_mode<LAT, XLBH_F16_SHIFTED, float>(p, s, mode, ws_words, flag, lam);
+        case XLBH_BF16_SHIFTED:
+          return launch_adjoint_mode<LAT, XLBH_BF16_SHIFTED, float>(p, s, mode, ws_words, flag, lam);
       }
     } else if (compute == XLBH_COMPUTE_F64) {
       switch (store) {
-        case XLBH_F64: return launch_adjoint<LAT, XLBH_F64, double>(p, s);
-        case XLBH_F32: return launch_adjoint<LAT, XLBH_F32, double>(p, s);
+        case XLBH_F64: return launch_adjoint_mode<LAT, XLBH_F64, double>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F32: return launch_adjoint_mode<LAT, XLBH_F32, double>(p, s, mode, ws_words, flag, lam);
       }
     }
     set_error("unsupported (store, compute) combination for the adjoint step");
@@ -441,6 +748,10 @@ static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam
   p.table = bc_table;
   p.bc_cells = bc_cells;
   p.n_bc_cells = n_bc_cells;
+  p.d_own = nullptr;
+  p.d_nb_in = nullptr;
+  p.d_nb_out = nullptr;
+  p.nb_flag = nullptr;
   p.force_field = force_field;
   p.grad_force = grad_force;
   p.force_mode = force_mode;
@@ -456,7 +767,7 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
                                 const xlbh_bc_desc* bc_table, const int32_t* bc_cells, int n_bc_cells,
                                 int force_mode, const double* force_uniform,
                                 const void* force_field, double* grad_force, int nx, int ny, int nz, double omega,
-                                int timestep0, int n_steps, int recompute, void* stream) {
+                                int timestep0, int n_steps, void* workspace, int flags, void* stream) {
   clear_error();
   if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
                                ny, nz))
@@ -472,6 +783,7 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
     return j == 0 ? f_ckpt : static_cast<const void*>(static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes);
   };
   // recompute f^{t0+1} .. f^{t0+n-1} (unless the caller recorded them, e.g. with xlbh_record)
+  const bool recompute = (flags & XLBH_ADJ_RECOMPUTE) != 0;
   for (int j = 1; recompute && j < n_steps; ++j) {
     if (int e = xlbh_step(lattice, XLBH_BGK, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
                           missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega,
@@ -480,16 +792,52 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
   }
   void* cur = lam_a;
   void* nxt = lam_b;
+  const hipStream_t s = (hipStream_t)stream;
+  // fused path: workspace = d_own | d_nb[0] | d_nb[1] (each Q N compute values) | nb_flag (N bytes)
+  const size_t qn = (size_t)lattice_q(lattice) * (size_t)nx * ny * nz;
+  const size_t cb = compute == XLBH_COMPUTE_F64 ? 8 : 4;
+  char* ws = static_cast<char*>(workspace);
+  uint8_t* flag = ws ? reinterpret_cast<uint8_t*>(ws + 3 * qn * cb) : nullptr;
+  auto fused_params = [&](const void* f, const void* lo, void* li, int t) {
+    AdjParams p = adjoint_params(f, lo, li, bc_mask, missing_bits, aux, bc_table, bc_cells, n_bc_cells, force_mode,
+                                 force_uniform, force_field, grad_force, nx, ny, nz, omega);
+    p.d_own = ws;
+    p.d_nb_out = ws + (1 + (t & 1)) * qn * cb;
+    p.d_nb_in = ws + (1 + ((t + 1) & 1)) * qn * cb;
+    p.nb_flag = flag;
+    return p;
+  };
+  if (ws && (flags & XLBH_ADJ_BEGIN)) {
+    const AdjParams p = fused_params(f_ckpt, lam_a, lam_b, timestep0);
+    const uint64_t words = (3 * qn * cb + (size_t)nx * ny * nz + 3) / 4;
+    if (int e = adjoint_dispatch(lattice, store, compute, p, s, ADJ_BEGIN, words, flag)) return e;
+  }
   for (int j = n_steps - 1; j >= 0; --j) {
-    const AdjParams p = adjoint_params(state(j), cur, nxt, bc_mask, missing_bits, aux, bc_table, bc_cells,
-                                       n_bc_cells, force_mode,
-                                       force_uniform, force_field, grad_force, nx, ny, nz, omega);
-    if (int e = adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream)) return e;
+    if (ws) {
+      const AdjParams p = fused_params(state(j), cur, nxt, timestep0 + j);
+      if (int e = adjoint_dispatch(lattice, store, compute, p, s, ADJ_FUSED)) return e;
+    } else {
+      const AdjParams p = adjoint_params(state(j), cur, nxt, bc_mask, missing_bits, aux, bc_table, bc_cells,
+                                         n_bc_cells, force_mode, force_uniform, force_field, grad_force, nx, ny, nz,
+                                         omega);
+      if (int e = adjoint_dispatch(lattice, store, compute, p, s)) return e;
+    }
     void* t = cur;
     cur = nxt;
     nxt = t;
   }
+  if (ws && (flags & XLBH_ADJ_END)) {
+    const AdjParams p = fused_params(f_ckpt, nullptr, nullptr, timestep0);
+    if (int e = adjoint_dispatch(lattice, store, compute, p, s, ADJ_FLUSH, 0, nullptr, cur)) return e;
+  }
   return 0;
+}
+
+extern "C" int64_t xlbh_adjoint_workspace_bytes(int lattice, int compute, int nx, int ny, int nz) {
+  if (!valid_lattice(lattice) || nx <= 0 || ny <= 0 || nz <= 0) return -1;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const int64_t cb = compute == XLBH_COMPUTE_F64 ? 8 : 4;
+  return ((3 * (int64_t)lattice_q(lattice) * n * cb + n + 255) / 256) * 256;
 }
 
 extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void* f_in, const void* lam_out,

@@ -285,13 +285,21 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.mark_written(states)
         return states
 
+    def adjoint_workspace(self, bc_mask):
+        """A workspace for the fused reverse step (see ``adjoint_run``)."""
+        nx, ny, nz = field_extents(bc_mask)
+        nbytes = int(_lib.lib().xlbh_adjoint_workspace_bytes(self.lattice_id, self.compute_id, nx, ny, nz))
+        return torch.empty(nbytes, dtype=torch.uint8, device=bc_mask.device)
+
     def adjoint_run(self, f_ckpt, states, lam_a, lam_b, bc_mask, missing_mask, omega, num_steps, timestep=0,
-                    grad_force=None, recompute=True):
+                    grad_force=None, recompute=True, workspace=None, begin=True, end=True):
         """Reverse sweep over one checkpoint segment below the ABI (xlbh_adjoint_run): recompute the
         ``num_steps - 1`` states after ``f_ckpt`` into ``states`` ((num_steps - 1, q, ...) in the
         store dtype), then apply the adjoint steps in reverse starting from lam_a = dL/df at the
         segment end.  Returns the buffer (lam_a or lam_b) that holds dL/df_ckpt.  With
-        ``recompute=False`` the states are taken as recorded by ``record``.  BGK only."""
+        ``recompute=False`` the states are taken as recorded by ``record``.  A ``workspace`` (from
+        ``adjoint_workspace``) selects the fused one-kernel reverse step; a sweep over several
+        segments shares it, with ``begin`` on its first call and ``end`` on its last.  BGK only."""
         if self.collision_type != "BGK":
             raise NotImplementedError("the adjoint step is implemented for BGK collision")
         if self._has_solid:
@@ -310,16 +318,22 @@ class IncompressibleNavierStokesStepper(Stepper):
                 raise ValueError("states must be a contiguous (num_steps - 1, q, nx, ny, nz) store-dtype tensor")
         if grad_force is not None:
             expect_field(grad_force, self.velocity_set.d, ext, "grad_force", torch.float64)
+        flags = (_lib.ADJ_RECOMPUTE if recompute else 0)
+        if workspace is not None:
+            need = int(_lib.lib().xlbh_adjoint_workspace_bytes(self.lattice_id, self.compute_id, nx, ny, nz))
+            if workspace.dtype != torch.uint8 or workspace.numel() < need or not workspace.is_contiguous():
+                raise ValueError(f"workspace must be a contiguous uint8 tensor of >= {need} bytes")
+            flags |= (_lib.ADJ_BEGIN if begin else 0) | (_lib.ADJ_END if end else 0)
         _lib.check(
             _lib.lib().xlbh_adjoint_run(
                 self.lattice_id, self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_ckpt), _lib.ptr(states), _lib.ptr(lam_a), _lib.ptr(lam_b), *masks, *self._adjoint_cells(),
                 force_mode, fu_ptr, _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, float(omega),
-                int(timestep), int(num_steps), int(bool(recompute)), _lib.stream_of(f_ckpt),
+                int(timestep), int(num_steps), _lib.ptr(workspace), flags, _lib.stream_of(f_ckpt),
             ),
             "IncompressibleNavierStokesStepper.adjoint_run",
         )
-        _lib.mark_written(states, lam_a, lam_b, grad_force)
+        _lib.mark_written(states, lam_a, lam_b, grad_force, workspace)
         return lam_a if num_steps % 2 == 0 else lam_b
 
     def run(self, f_0, f_1, bc_mask, missing_mask, omega, num_steps, timestep=0):
