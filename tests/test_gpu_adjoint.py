@@ -91,11 +91,11 @@ class Case:
         _, u = macroscopic_torch(fT, self.vs)
         return (self.W * fT).sum() + 50.0 * (u * u).sum()
 
-    def gradients(self, steps, every=None, ckpt="store", overlap="auto"):
+    def gradients(self, steps, every=None, ckpt="store", overlap="auto", fused="auto"):
         f0 = self.f0.clone().requires_grad_(True)
         ff = self.ff.clone().requires_grad_(True) if self.ff is not None else None
         fT = rollout(self.st, f0, self.bc_mask, self.missing, self.omega, steps, force_field=ff,
-                     checkpoint_every=every, checkpoint_dtype=ckpt, overlap=overlap)
+                     checkpoint_every=every, checkpoint_dtype=ckpt, overlap=overlap, fused=fused)
         self.loss(fT).backward()
         return f0.grad.double(), (ff.grad.double() if ff is not None else None)
 
@@ -197,8 +197,9 @@ def test_checkpoint_spacing_invariance():
     steps = 13
     c = Case("channel2d_zouhe", P.FP64FP64, seed=4)
     ref_f, ref_g = c.gradients(steps, every=steps)
-    for every, overlap in ((1, True), (3, True), (None, True), (3, False), (None, False)):
-        g_f, g_g = c.gradients(steps, every=every, overlap=overlap)
+    for every, overlap, fused in ((1, True, True), (3, True, True), (None, True, False), (3, False, True),
+                                  (None, False, False)):
+        g_f, g_g = c.gradients(steps, every=every, overlap=overlap, fused=fused)
         assert ((g_f - ref_f).norm() / ref_f.norm()).item() <= 1e-12
         assert ((g_g - ref_g).norm() / ref_g.norm()).item() <= 1e-12
     c32 = Case("channel2d_zouhe", P.FP32FP32, seed=4)
@@ -252,8 +253,8 @@ def test_fused_and_split_reverse_agree(name):
     regularized Jacobian by the hand-written one-pass adjoint) against the two-kernel reverse step
     (atomics, Jacobian by forward-mode dual numbers), over a checkpointed rollout in FP64."""
     c = Case(name, P.FP64FP64, seed=6)
-    f_fused, g_fused = c.gradients(9, every=4, overlap=True)
-    f_split, g_split = c.gradients(9, every=4, overlap=False)
+    f_fused, g_fused = c.gradients(9, every=4, fused=True)
+    f_split, g_split = c.gradients(9, every=4, fused=False)
     assert ((f_fused - f_split).norm() / f_split.norm()).item() <= 1e-12
     if g_split is not None:
         assert ((g_fused - g_split).norm() / g_split.norm()).item() <= 1e-12
