@@ -16,7 +16,8 @@ Also reported on the same line:
 * adjoint -- backward (checkpoint recompute + adjoint kernels) over forward time per step for the
   differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50) and the 256^3 cavity;
 * kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, with the momentum-exchange drag
-  every 100 steps inside the timed loop.
+  every 100 steps inside the timed loop;
+* mesh_masker -- surface voxelisation of an 81,920-triangle sphere mesh into the same grid.
 roofline.achieved = algorithmic bytes per launch (2 Q S_store per cell) / average kernel duration
 from HIP events on the launch stream; roofline.traffic = FETCH_SIZE + WRITE_SIZE per launch from
 rocprofv3 (profiles/traffic_step_d3q19_cavity256.json, written by --summarize-pmc).
@@ -369,6 +370,35 @@ def c1_leg(xa, torch, with_cpu=True):
     return res
 
 
+def mesh_leg(xa, torch, shape=(256, 128, 128), subdivisions=6, reps=5):
+    """MeshBoundaryMasker: surface voxelisation of an icosphere (20 * 4^k triangles, radius 30) into
+    the C2 wind-tunnel grid, best of `reps` (HIP events)."""
+    from xlb_amd.workloads import _setup, icosphere
+
+    vs, grid = _setup("D3Q19", xa.PrecisionPolicy.FP32FP32, shape)
+    tris = icosphere((shape[0] / 4, shape[1] / 2, shape[2] / 2), 30.0, subdivisions)
+    masker = xa.operator.boundary_masker.MeshBoundaryMasker(vs, xa.PrecisionPolicy.FP32FP32, xa.ComputeBackend.MI355X)
+    best = float("inf")
+    for _ in range(reps):
+        bc = xa.operator.boundary_condition.HalfwayBounceBackBC(mesh_vertices=tris.reshape(-1, 3))
+        bc_mask = torch.zeros((1,) + shape, dtype=torch.uint8, device="cuda")
+        missing = torch.zeros((vs.q,) + shape, dtype=torch.bool, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        masker(bc, bc_mask, missing)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    cells = int(np.prod(shape))
+    res = dict(workload=f"icosphere r=30, {tris.shape[0]} triangles, voxelised into {shape[0]}x{shape[1]}x{shape[2]} "
+                        "(D3Q19 masks), host call to masks ready",
+               ms=best * 1e3, solid_voxels=int((bc_mask == 255).sum().item()),
+               boundary_voxels=int((bc_mask == bc.id).sum().item()), mcells_per_s=cells / best / 1e6)
+    log(f"bench: mesh masker {res['ms']:.2f} ms for {tris.shape[0]} triangles")
+    del bc_mask, missing
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_leg(n, budget_s=12.0, max_steps=40):
     """The C++/OpenMP oracle on the headline workload, FP32: a bounded sample of steps."""
     from oracle import oracle as orc
@@ -524,6 +554,7 @@ def main():
         extras["precision_headline"] = headline_precision_leg(xa, torch, 128 if args.quick else n, 1000)
         extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
         extras["kbc_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick)
+        extras["mesh_masker"] = mesh_leg(xa, torch, subdivisions=4 if args.quick else 6)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline (oracle) ...")

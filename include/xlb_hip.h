@@ -225,6 +225,18 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
 int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask,
                        double* out, void* workspace, int nx, int ny, int nz, void* stream);
 
+/* Surface voxelisation of a triangle mesh into the masks, replacing the reference's
+ * MeshBoundaryMasker (xlb/operator/boundary_masker/mesh_boundary_masker.py:49-236).
+ *   triangles: (T, 3, 3) float32 device array, the vertices of each triangle in lattice units
+ *              (voxel (i,j,k) is the unit box [i,i+1] x [j,j+1] x [k,k+1]).
+ * A voxel some triangle touches (Schwarz-Seidel triangle/box test) gets bc_mask = 255; any other
+ * voxel whose neighbour x + c_l is touched gets bc_mask = bc_id and missing[opp l] = 1.  Other
+ * cells are left as they are.  3-D lattices only.  `workspace` holds
+ * xlbh_mesh_workspace_bytes(nx, ny, nz) bytes (a solid grid with one halo voxel per side). */
+int64_t xlbh_mesh_workspace_bytes(int nx, int ny, int nz);
+int xlbh_mesh_mask(int lattice, const float* triangles, int64_t n_triangles, int bc_id, uint8_t* bc_mask,
+                   uint8_t* missing_bool, void* workspace, int nx, int ny, int nz, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Reverse mode of the fused step (BGK, all boundary kinds).  No counterpart function in the
  * reference, whose gradients come from Warp's tape replaying generated adjoint kernels

@@ -53,6 +53,8 @@ def lib():
         L.orc_collide.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, d, i, vp, vp, i, i, i]
         L.orc_apply_bc.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, i, i, i]
         L.orc_mask_indices.argtypes = [i, vp, vp, i, i, vp, vp, i, i, i]
+        L.orc_mesh_mask.argtypes = [i, vp, ctypes.c_int64, i, vp, vp, i, i, i]
+        L.orc_tri_box.argtypes = [vp, vp]
         L.orc_momentum_transfer.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, i, i, vp]
         L.orc_lattice.argtypes = [i, vp, vp, vp, vp]
         L.orc_set_threads.argtypes = [i]
@@ -175,6 +177,22 @@ def mask_indices(lat, indices, interior, bc_id, bc_mask, missing):
     n = idx.shape[1]
     assert lib().orc_mask_indices(lat, _p(idx), _p(np.ascontiguousarray(interior, dtype=np.uint8)), n, int(bc_id),
                                   _p(bc_mask), _p(missing), nx, ny, nz) == 0
+
+
+def mesh_mask(lat, triangles, bc_id, bc_mask, missing):
+    """MeshBoundaryMasker restatement, in place on bc_mask (1,nx,ny,nz) uint8 and missing
+    (q,nx,ny,nz) uint8; triangles (T, 3, 3)."""
+    nx, ny, nz = _dims(bc_mask)
+    tri = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1, 9)
+    assert lib().orc_mesh_mask(lat, _p(tri), ctypes.c_int64(tri.shape[0]), int(bc_id), _p(bc_mask), _p(missing),
+                               nx, ny, nz) == 0
+
+
+def tri_box(tri9, p):
+    """Schwarz-Seidel triangle / unit-box test of the oracle (box minimum corner p)."""
+    t = np.ascontiguousarray(tri9, dtype=np.float32).reshape(9)
+    q = np.ascontiguousarray(p, dtype=np.float32).reshape(3)
+    return bool(lib().orc_tri_box(_p(t), _p(q)))
 
 
 def momentum_transfer(lat, store, compute, bc, f0, bc_mask, missing, aux=None):

@@ -851,6 +851,116 @@ int orc_mask_indices(int lat, const int32_t* indices, const uint8_t* interior, i
   return 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// MeshBoundaryMasker (mesh_boundary_masker.py:49-236), voxel-parallel as the reference: a voxel
+// (unit box with minimum corner at its index) touched by any triangle is solid (255); otherwise
+// every direction l whose neighbour box (index + c_l) is touched sets missing[opp l] and the id.
+// Triangle / box test: bounding-box overlap (the reference's BVH query, :139), then Schwarz &
+// Seidel 2010, sec. 3.1 (plane overlap through the critical corners, then the three axis
+// projections with three edge functions each), FP32, fixed order.
+// ---------------------------------------------------------------------------------------------
+struct OrcTri {
+  float lo[3], hi[3], n[3], d1, d2, ne0[3][3], ne1[3][3], de[3][3];
+  bool valid;
+};
+
+static float orc_dot3(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+static OrcTri orc_tri_prepare(const float* v) {
+  OrcTri t{};
+  const float* vv[3] = {v, v + 3, v + 6};
+  float e[3][3], w[3], cr[3];
+  for (int a = 0; a < 3; ++a) {
+    e[0][a] = vv[1][a] - vv[0][a];
+    e[1][a] = vv[2][a] - vv[1][a];
+    e[2][a] = vv[0][a] - vv[2][a];
+    w[a] = vv[2][a] - vv[0][a];
+    t.lo[a] = std::min(std::min(vv[0][a], vv[1][a]), vv[2][a]);
+    t.hi[a] = std::max(std::max(vv[0][a], vv[1][a]), vv[2][a]);
+  }
+  cr[0] = e[0][1] * w[2] - e[0][2] * w[1];
+  cr[1] = e[0][2] * w[0] - e[0][0] * w[2];
+  cr[2] = e[0][0] * w[1] - e[0][1] * w[0];
+  const float len = std::sqrt(orc_dot3(cr, cr));
+  t.valid = len > 0.0f;
+  for (int a = 0; a < 3; ++a) t.n[a] = t.valid ? cr[a] / len : 0.0f;
+  float c[3], cm[3];
+  for (int a = 0; a < 3; ++a) {
+    const float k = t.n[a] > 0.0f ? 1.0f : 0.0f;
+    c[a] = k - vv[0][a];
+    cm[a] = (1.0f - k) - vv[0][a];
+  }
+  t.d1 = orc_dot3(t.n, c);
+  t.d2 = orc_dot3(t.n, cm);
+  for (int p = 0; p < 3; ++p) {
+    const int a0 = p, a1 = (p + 1) % 3, a2 = (p + 2) % 3;
+    const float sg = t.n[a2] < 0.0f ? -1.0f : 1.0f;
+    for (int i = 0; i < 3; ++i) {
+      const float n0 = -sg * e[i][a1];
+      const float n1 = sg * e[i][a0];
+      t.ne0[p][i] = n0;
+      t.ne1[p][i] = n1;
+      t.de[p][i] = (-(n0 * vv[i][a0] + n1 * vv[i][a1]) + std::max(0.0f, n0)) + std::max(0.0f, n1);
+    }
+  }
+  return t;
+}
+
+static bool orc_tri_box_pre(const OrcTri& t, const float* p) {
+  if (!t.valid) return false;
+  for (int a = 0; a < 3; ++a)  // bounding boxes overlap (the reference's BVH query, :139)
+    if (t.lo[a] > p[a] + 1.0f || t.hi[a] < p[a]) return false;
+  const float np = orc_dot3(t.n, p);
+  if ((np + t.d1) * (np + t.d2) > 0.0f) return false;
+  for (int q = 0; q < 3; ++q) {
+    const int a0 = q, a1 = (q + 1) % 3;
+    for (int i = 0; i < 3; ++i)
+      if ((t.ne0[q][i] * p[a0] + t.ne1[q][i] * p[a1]) + t.de[q][i] < 0.0f) return false;
+  }
+  return true;
+}
+
+// Single test, exported for checking against an independent overlap test.
+int orc_tri_box(const float* tri9, const float* p) { return orc_tri_box_pre(orc_tri_prepare(tri9), p) ? 1 : 0; }
+
+int orc_mesh_mask(int lat, const float* tri, int64_t n_tri, int id, uint8_t* bc_mask, uint8_t* missing, int nx, int ny,
+                  int nz) {
+  const Lattice& L = g_lat[lat];
+  const size_t N = (size_t)nx * ny * nz;
+  std::vector<OrcTri> pre((size_t)n_tri);
+  for (int64_t t = 0; t < n_tri; ++t) pre[(size_t)t] = orc_tri_prepare(tri + 9 * t);
+  const int hx = nx + 2, hy = ny + 2, hz = nz + 2;
+  std::vector<uint8_t> solid((size_t)hx * hy * hz, 0);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int x = -1; x <= nx; ++x)
+    for (int y = -1; y <= ny; ++y)
+      for (int z = -1; z <= nz; ++z) {
+        const float p[3] = {(float)x, (float)y, (float)z};
+        for (int64_t t = 0; t < n_tri; ++t)
+          if (orc_tri_box_pre(pre[(size_t)t], p)) {
+            solid[((size_t)(x + 1) * hy + (y + 1)) * hz + (z + 1)] = 1;
+            break;
+          }
+      }
+  auto sol = [&](int x, int y, int z) { return solid[((size_t)(x + 1) * hy + (y + 1)) * hz + (z + 1)] != 0; };
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const size_t cell = ((size_t)x * ny + y) * nz + z;
+        if (sol(x, y, z)) {
+          bc_mask[cell] = 255;
+          continue;
+        }
+        for (int l = 1; l < L.q; ++l) {
+          if (sol(x + L.c[0][l], y + L.c[1][l], z + L.c[2][l])) {
+            bc_mask[cell] = (uint8_t)id;
+            missing[(size_t)L.opp[l] * N + cell] = 1;
+          }
+        }
+      }
+  return 0;
+}
+
 // MomentumTransfer (momentum_transfer.py:108-176), FP64 accumulation over cells.
 int orc_momentum_transfer(int lat, int store, int compute, const OrcBc* bc, const void* f0, const uint8_t* bc_mask,
                           const uint8_t* missing, const double* aux, int nx, int ny, int nz, double* out) {

@@ -23,7 +23,7 @@ from xlb_amd.default_config import DefaultConfig
 from xlb_amd.helper.check_boundary_overlaps import check_bc_overlaps
 from xlb_amd.helper.nse_solver import create_nse_fields
 from xlb_amd.operator.boundary_condition.boundary_condition import ImplementationStep
-from xlb_amd.operator.boundary_masker import IndicesBoundaryMasker
+from xlb_amd.operator.boundary_masker import IndicesBoundaryMasker, MeshBoundaryMasker
 from xlb_amd.operator.boundary_masker.packing import cells_with_id, packed_missing
 from xlb_amd.operator.collision import BGK, KBC, ForcedCollision, SmagorinskyLESBGK
 from xlb_amd.operator.equilibrium import QuadraticEquilibrium
@@ -74,9 +74,6 @@ class IncompressibleNavierStokesStepper(Stepper):
         self.stream = Stream(self.velocity_set, self.precision_policy, self.compute_backend)
         self.equilibrium = QuadraticEquilibrium(self.velocity_set, self.precision_policy, self.compute_backend)
         self.macroscopic = Macroscopic(self.velocity_set, self.precision_policy, self.compute_backend)
-        for bc in self.boundary_conditions:
-            if bc.mesh_vertices is not None:
-                raise NotImplementedError("mesh-based boundary conditions (MeshBoundaryMasker) are not implemented by the MI355X backend")
         self._state_key = None
         self._bits = None
         self._table = None
@@ -106,16 +103,25 @@ class IncompressibleNavierStokesStepper(Stepper):
 
     @classmethod
     def _process_boundary_conditions(cls, boundary_conditions, bc_mask, missing_mask):
-        """Overlap check and indices masker (nse_stepper.py:100-127)."""
+        """Overlap check, indices masker, then the mesh masker for 3-D mesh BCs (nse_stepper.py:100-127)."""
         check_bc_overlaps(boundary_conditions, DefaultConfig.velocity_set.d, DefaultConfig.default_backend)
         indices_masker = IndicesBoundaryMasker(
             velocity_set=DefaultConfig.velocity_set,
             precision_policy=DefaultConfig.default_precision_policy,
             compute_backend=DefaultConfig.default_backend,
         )
+        bc_with_vertices = [bc for bc in boundary_conditions if bc.mesh_vertices is not None]
         bc_with_indices = [bc for bc in boundary_conditions if bc.indices is not None]
         if bc_with_indices:
             bc_mask, missing_mask = indices_masker(bc_with_indices, bc_mask, missing_mask)
+        if DefaultConfig.velocity_set.d == 3 and bc_with_vertices:
+            mesh_masker = MeshBoundaryMasker(
+                velocity_set=DefaultConfig.velocity_set,
+                precision_policy=DefaultConfig.default_precision_policy,
+                compute_backend=DefaultConfig.default_backend,
+            )
+            for bc in bc_with_vertices:
+                bc_mask, missing_mask = mesh_masker(bc, bc_mask, missing_mask)
         return bc_mask, missing_mask
 
     def _initialize_auxiliary_data(self, boundary_conditions, f_0, f_1, bc_mask, missing_mask):

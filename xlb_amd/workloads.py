@@ -197,3 +197,22 @@ def decaying_turbulence_3d(n, policy, omega=1.8):
     st = xa.operator.stepper.IncompressibleNavierStokesStepper(grid, [], collision_type="BGK")
     f0, f1, bm, mm = st.prepare_fields()
     return Workload(st, f0, f1, bm, mm, omega)
+
+
+def icosphere(center, radius, subdivisions):
+    """Triangle soup (20 * 4^k, 3, 3) of a subdivided icosahedron projected on a sphere, the kind of
+    mesh the reference's wind tunnel loads from STL (windtunnel_3d.py:76-95)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    tris = np.array([[v[a], v[b], v[c]] for a, b, c in f], dtype=float)
+    tris /= np.linalg.norm(tris, axis=-1, keepdims=True)
+    for _ in range(subdivisions):
+        a, b, c = tris[:, 0], tris[:, 1], tris[:, 2]
+        ab, bc, ca = (a + b) / 2, (b + c) / 2, (c + a) / 2
+        ab, bc, ca = (x / np.linalg.norm(x, axis=-1, keepdims=True) for x in (ab, bc, ca))
+        tris = np.concatenate([np.stack(s, 1) for s in ((a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca))])
+    return tris * radius + np.asarray(center, float)
