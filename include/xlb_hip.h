@@ -237,6 +237,19 @@ int64_t xlbh_mesh_workspace_bytes(int nx, int ny, int nz);
 int xlbh_mesh_mask(int lattice, const float* triangles, int64_t n_triangles, int bc_id, uint8_t* bc_mask,
                    uint8_t* missing_bool, void* workspace, int nx, int ny, int nz, void* stream);
 
+/* Post-processing of a velocity field u (3, nx, ny, nz) of type `field` (F32 or F64): over the
+ * interior cells, skipping every cell with a non-zero boundary id among its six face neighbours
+ * (bc_mask may be NULL), central differences give
+ *   xlbh_vorticity:   vorticity (3, N) = curl u and magnitude (1, N)
+ *                     (xlb/operator/postprocess/vorticity.py:30-80);
+ *   xlbh_q_criterion: norm_mu (1, N) = |curl u| and q (1, N) = (|Omega|^2 - |S|^2) / 2
+ *                     (q_criterion.py:35-132).
+ * Cells not computed keep their previous output values. */
+int xlbh_vorticity(int field, const void* u, const uint8_t* bc_mask, void* vorticity, void* magnitude,
+                   int nx, int ny, int nz, void* stream);
+int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* norm_mu, void* q,
+                     int nx, int ny, int nz, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Reverse mode of the fused step (BGK, all boundary kinds).  No counterpart function in the
  * reference, whose gradients come from Warp's tape replaying generated adjoint kernels

@@ -1,0 +1,138 @@
+// Post-processing stencils on the velocity field: vorticity and Q-criterion, replacing the
+// reference's Vorticity (xlb/operator/postprocess/vorticity.py:30-80) and QCriterion
+// (q_criterion.py:35-132) kernels.
+//
+// Both run over the interior cells 1 .. n-2 of each axis, skip a cell when any of its six face
+// neighbours has a non-zero boundary id (the cell's own id is not examined), and leave skipped
+// and edge cells untouched.  Derivatives are central differences (a - b) / 2; the arithmetic
+// follows the reference's expressions term by term in the field's precision.
+#include "common.hpp"
+
+namespace xlbh {
+
+template <class T>
+struct Grad {
+  T xx, xy, xz, yx, yy, yz, zx, zy, zz;  // d u_a / d b as u_a_db
+};
+
+// Returns false when the cell is on the domain edge or next to a boundary cell.
+template <class T>
+__device__ __forceinline__ bool central_gradient(const T* u, const uint8_t* bc_mask, int x, int y, int z, int nx,
+                                                 int ny, int nz, uint64_t N, Grad<T>& g) {
+  if (x < 1 || y < 1 || z < 1 || x > nx - 2 || y > ny - 2 || z > nz - 2) return false;
+  auto at = [&](int a, int b, int c) { return ((uint64_t)a * ny + b) * nz + c; };
+  if (bc_mask) {
+    if (bc_mask[at(x + 1, y, z)] | bc_mask[at(x, y + 1, z)] | bc_mask[at(x, y, z + 1)] | bc_mask[at(x - 1, y, z)] |
+        bc_mask[at(x, y - 1, z)] | bc_mask[at(x, y, z - 1)])
+      return false;
+  }
+  const T* ux = u;
+  const T* uy = u + N;
+  const T* uz = u + 2 * N;
+  const T h = T(2);
+  g.xx = (ux[at(x + 1, y, z)] - ux[at(x - 1, y, z)]) / h;
+  g.xy = (ux[at(x, y + 1, z)] - ux[at(x, y - 1, z)]) / h;
+  g.xz = (ux[at(x, y, z + 1)] - ux[at(x, y, z - 1)]) / h;
+  g.yx = (uy[at(x + 1, y, z)] - uy[at(x - 1, y, z)]) / h;
+  g.yy = (uy[at(x, y + 1, z)] - uy[at(x, y - 1, z)]) / h;
+  g.yz = (uy[at(x, y, z + 1)] - uy[at(x, y, z - 1)]) / h;
+  g.zx = (uz[at(x + 1, y, z)] - uz[at(x - 1, y, z)]) / h;
+  g.zy = (uz[at(x, y + 1, z)] - uz[at(x, y - 1, z)]) / h;
+  g.zz = (uz[at(x, y, z + 1)] - uz[at(x, y, z - 1)]) / h;
+  return true;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_vorticity(const T* __restrict__ u, const uint8_t* __restrict__ bc_mask,
+                                                   T* __restrict__ vort, T* __restrict__ mag, Sweep sw) {
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const uint64_t N = (uint64_t)sw.nx * sw.ny * sw.nz;
+  Grad<T> g;
+  if (!central_gradient(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
+  const uint64_t cell = ((uint64_t)x * sw.ny + y) * sw.nz + z;
+  const T vx = g.zy - g.yz;
+  const T vy = g.xz - g.zx;
+  const T vz = g.yx - g.xy;
+  vort[cell] = vx;
+  vort[N + cell] = vy;
+  vort[2 * N + cell] = vz;
+  mag[cell] = sqrt(vx * vx + vy * vy + vz * vz);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_q_criterion(const T* __restrict__ u, const uint8_t* __restrict__ bc_mask,
+                                                     T* __restrict__ norm_mu, T* __restrict__ q, Sweep sw) {
+  int x, y, z;
+  if (!sweep_cell(sw, x, y, z)) return;
+  const uint64_t N = (uint64_t)sw.nx * sw.ny * sw.nz;
+  Grad<T> g;
+  if (!central_gradient(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
+  const uint64_t cell = ((uint64_t)x * sw.ny + y) * sw.nz + z;
+  const T mx = g.zy - g.yz, my = g.xz - g.zx, mz = g.yx - g.xy;
+  const T mu = sqrt(mx * mx + my * my + mz * mz);
+  const T half = T(0.5);
+  const T s00 = g.xx, s01 = half * (g.xy + g.yx), s02 = half * (g.xz + g.zx);
+  const T s11 = g.yy, s12 = half * (g.yz + g.zy), s22 = g.zz;
+  const T ss = s00 * s00 + s01 * s01 + s02 * s02 + s01 * s01 + s11 * s11 + s12 * s12 + s02 * s02 + s12 * s12 + s22 * s22;
+  const T o01 = half * (g.xy - g.yx), o02 = half * (g.xz - g.zx), o12 = half * (g.yz - g.zy);
+  const T zero = T(0);
+  const T oo = zero * zero + o01 * o01 + o02 * o02 + (-o01) * (-o01) + zero * zero + o12 * o12 + (-o02) * (-o02) +
+               (-o12) * (-o12) + zero * zero;
+  norm_mu[cell] = mu;
+  q[cell] = half * (oo - ss);
+}
+
+}  // namespace xlbh
+
+using namespace xlbh;
+
+extern "C" {
+
+int xlbh_vorticity(int field, const void* u, const uint8_t* bc_mask, void* vorticity, void* magnitude, int nx, int ny,
+                   int nz, void* stream) {
+  clear_error();
+  XLBH_REQUIRE(nx > 2 && ny > 2 && nz > 2, "vorticity needs a 3-D grid of at least 3 cells per axis");
+  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 32), "more than 2^32 - 1 cells per field");
+  XLBH_REQUIRE(nx <= 65535, "nx must be <= 65535");
+  XLBH_REQUIRE(u && vorticity && magnitude, "vorticity needs u, vorticity and magnitude");
+  const Sweep sw = make_sweep(nx, ny, nz);
+  hipStream_t s = (hipStream_t)stream;
+  if (field == XLBH_FIELD_F32)
+    hipLaunchKernelGGL(k_vorticity<float>, sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
+                       (float*)vorticity, (float*)magnitude, sw);
+  else if (field == XLBH_FIELD_F64)
+    hipLaunchKernelGGL(k_vorticity<double>, sweep_grid(sw), dim3(256), 0, s, (const double*)u, bc_mask,
+                       (double*)vorticity, (double*)magnitude, sw);
+  else {
+    set_error("vorticity supports float32 and float64 fields");
+    return 1;
+  }
+  XLBH_LAUNCH_CHECK();
+  return 0;
+}
+
+int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* norm_mu, void* q, int nx, int ny, int nz,
+                     void* stream) {
+  clear_error();
+  XLBH_REQUIRE(nx > 2 && ny > 2 && nz > 2, "q_criterion needs a 3-D grid of at least 3 cells per axis");
+  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 32), "more than 2^32 - 1 cells per field");
+  XLBH_REQUIRE(nx <= 65535, "nx must be <= 65535");
+  XLBH_REQUIRE(u && norm_mu && q, "q_criterion needs u, norm_mu and q");
+  const Sweep sw = make_sweep(nx, ny, nz);
+  hipStream_t s = (hipStream_t)stream;
+  if (field == XLBH_FIELD_F32)
+    hipLaunchKernelGGL(k_q_criterion<float>, sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
+                       (float*)norm_mu, (float*)q, sw);
+  else if (field == XLBH_FIELD_F64)
+    hipLaunchKernelGGL(k_q_criterion<double>, sweep_grid(sw), dim3(256), 0, s, (const double*)u, bc_mask,
+                       (double*)norm_mu, (double*)q, sw);
+  else {
+    set_error("q_criterion supports float32 and float64 fields");
+    return 1;
+  }
+  XLBH_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"
