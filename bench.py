@@ -17,7 +17,10 @@ Also reported on the same line:
   differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50) and the 256^3 cavity;
 * kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, with the momentum-exchange drag
   every 100 steps inside the timed loop;
-* mesh_masker -- surface voxelisation of an 81,920-triangle sphere mesh into the same grid.
+* mesh_masker -- surface voxelisation of an 81,920-triangle sphere mesh into the same grid;
+* les_cavity -- the LES lid-driven cavity of examples/out_of_core/lid_driven_cavity.py (D3Q19
+  SmagorinskyLESBGK, Cs = 0.17, 256^3, lid u = 0.06, tau = 0.501, FP32), with the Q-criterion
+  of its velocity field (examples' RenderQCriterionSubroutine) timed on the final state.
 roofline.achieved = algorithmic bytes per launch (2 Q S_store per cell) / average kernel duration
 from HIP events on the launch stream; roofline.traffic = FETCH_SIZE + WRITE_SIZE per launch from
 rocprofv3 (profiles/traffic_step_d3q19_cavity256.json, written by --summarize-pmc).
@@ -300,6 +303,46 @@ def kbc_sphere_leg(xa, torch, steps=500, quick=False):
     return res
 
 
+def les_cavity_leg(xa, torch, n, steps, warmup):
+    """SmagorinskyLESBGK lid-driven cavity (out_of_core/lid_driven_cavity.py:38-47 defaults) and
+    the Q-criterion post-processing of its final velocity field."""
+    from xlb_amd.workloads import lid_driven_cavity_3d
+
+    wl = lid_driven_cavity_3d(n, xa.PrecisionPolicy.FP32FP32, lid_u=0.06, omega=1.0 / 0.501,
+                              collision="SmagorinskyLESBGK")
+    wall, ev_s, f0, f1 = time_run(torch, wl, steps, warmup)
+    cells = n**3
+    res = dict(workload=f"D3Q19 SmagorinskyLESBGK (Cs 0.17) lid-driven cavity {n}^3, lid u = 0.06, tau = 0.501, "
+                        f"FP32, {steps} steps", mlups=cells * steps / wall / 1e6, kernel_ms=ev_s / steps * 1e3,
+               achieved_gbs=152 * cells / (ev_s / steps) / 1e9)
+    res["frac"] = res["achieved_gbs"] / HBM_PEAK_GBS
+    res["finite"] = bool(torch.isfinite(f0).all().item())
+    g = wl.stepper.grid
+    rho, u = g.create_field(1), g.create_field(3)
+    xa.operator.macroscopic.Macroscopic()(f0, rho, u)
+    norm_mu, qf = g.create_field(1), g.create_field(1)
+    qc = xa.operator.postprocess.QCriterion()
+    qc(u, wl.bc_mask, norm_mu, qf)
+    reps = 20
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        qc(u, wl.bc_mask, norm_mu, qf)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    q_s = ev0.elapsed_time(ev1) / 1e3 / reps
+    # algorithmic bytes per cell: 3 velocity components + bc_mask in, |curl u| and Q out
+    res.update(q_criterion_ms=q_s * 1e3, q_criterion_gbs=(3 * 4 + 1 + 2 * 4) * cells / q_s / 1e9,
+               max_u=float(u.abs().max().item()), max_q=float(qf.max().item()))
+    log(f"bench: les cavity {res['mlups']:.0f} MLUPS ({res['achieved_gbs']:.0f} GB/s), "
+        f"q-criterion {res['q_criterion_ms']:.3f} ms ({res['q_criterion_gbs']:.0f} GB/s)")
+    del wl, f0, f1, rho, u, norm_mu, qf
+    torch.cuda.empty_cache()
+    return res
+
+
 def periodic_leg(xa, torch, n, steps, warmup):
     """The pure step kernel: D3Q19 BGK FP32 periodic box without boundary conditions."""
     from xlb_amd.workloads import periodic_box_3d
@@ -555,6 +598,7 @@ def main():
         extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
         extras["kbc_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick)
         extras["mesh_masker"] = mesh_leg(xa, torch, subdivisions=4 if args.quick else 6)
+        extras["les_cavity"] = les_cavity_leg(xa, torch, 128 if args.quick else n, args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline (oracle) ...")

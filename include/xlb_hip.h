@@ -30,7 +30,7 @@ enum { XLBH_D2Q9 = 0, XLBH_D3Q19 = 1, XLBH_D3Q27 = 2 };
 
 /* Collision operators (xlb/operator/collision/bgk.py, kbc.py). KBC: D2Q9 and D3Q27 only,
  * as in kbc.py:183-185. */
-enum { XLBH_BGK = 0, XLBH_KBC = 1 };
+enum { XLBH_BGK = 0, XLBH_KBC = 1, XLBH_SMAGORINSKY_BGK = 2 };
 
 /* Storage formats of population fields.  The *_SHIFTED formats store the deviation
  * f_l - w_l (no counterpart in the reference; its FP16 store is a plain cast,
@@ -107,13 +107,15 @@ int xlbh_abi_version(void);
  *                   (device, (D, N) in the compute type) for FIELD.
  *   omega         : relaxation rate.  timestep is accepted for interface parity (unused by
  *                   the in-scope boundary conditions, as in the reference).
+ *   collision_param: the Smagorinsky constant for XLBH_SMAGORINSKY_BGK
+ *                   (smagorinsky_les_bgk.py:19-26, default 0.17 there); ignored otherwise.
  * ------------------------------------------------------------------------------------- */
 int xlbh_step(int lattice, int collision, int store, int compute,
               const void* f_in, void* f_out,
               const uint8_t* bc_mask, const uint32_t* missing_bits,
               const void* aux, const xlbh_bc_desc* bc_table,
               int force_mode, const double* force_uniform, const void* force_field,
-              int nx, int ny, int nz, double omega, int timestep, void* stream);
+              int nx, int ny, int nz, double omega, double collision_param, int timestep, void* stream);
 
 /* n_steps fused steps with the buffer swap done in C (the caller's loop of
  * mlups_3d.py:78-80 moved below the ABI).  Step t reads f_a when t is even and f_b when t is
@@ -123,7 +125,8 @@ int xlbh_run(int lattice, int collision, int store, int compute,
              const uint8_t* bc_mask, const uint32_t* missing_bits,
              const void* aux, const xlbh_bc_desc* bc_table,
              int force_mode, const double* force_uniform, const void* force_field,
-             int nx, int ny, int nz, double omega, int timestep0, int n_steps, void* stream);
+             int nx, int ny, int nz, double omega, double collision_param, int timestep0, int n_steps,
+             void* stream);
 
 /* n_states steps from f_start, keeping every state: state j (the output of step t0 + j) is
  * written to states + j * (Q N S_store) bytes, for j = 0 .. n_states - 1.  The forward pass of a
@@ -132,7 +135,8 @@ int xlbh_run(int lattice, int collision, int store, int compute,
 int xlbh_record(int lattice, int collision, int store, int compute, const void* f_start, void* states,
                 const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
                 const xlbh_bc_desc* bc_table, int force_mode, const double* force_uniform,
-                const void* force_field, int nx, int ny, int nz, double omega, int timestep0,
+                const void* force_field, int nx, int ny, int nz, double omega, double collision_param,
+                int timestep0,
                 int n_states, void* stream);
 
 /* ---------------------------------------------------------------------------------------
@@ -159,7 +163,7 @@ int xlbh_moments(int lattice, int store, int compute, int field, int which,
  * kbc.py:268-347, forced_collision.py:41-104.  rho, u: (1,N), (D,N) in type `field`. */
 int xlbh_collide(int lattice, int collision, int store, int compute, int field,
                  const void* f, const void* feq, void* fout,
-                 const void* rho, const void* u, double omega,
+                 const void* rho, const void* u, double omega, double collision_param,
                  int force_mode, const double* force_uniform, const void* force_field,
                  int nx, int ny, int nz, void* stream);
 

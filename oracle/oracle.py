@@ -45,12 +45,12 @@ def lib():
             build()
         L = ctypes.CDLL(LIB_PATH)
         vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
-        L.orc_step.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, i, i, vp, vp, i, i, i, d]
+        L.orc_step.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, i, i, vp, vp, i, i, i, d, d]
         L.orc_aux_init.argtypes = [i, vp, vp, i, vp, i, i, i]
         L.orc_equilibrium.argtypes = [i, i, i, vp, vp, vp, i, i, i]
         L.orc_macroscopic.argtypes = [i, i, i, vp, vp, vp, i, i, i]
         L.orc_stream.argtypes = [i, i, vp, vp, i, i, i]
-        L.orc_collide.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, d, i, vp, vp, i, i, i]
+        L.orc_collide.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, d, d, i, vp, vp, i, i, i]
         L.orc_apply_bc.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, i, i, i]
         L.orc_mask_indices.argtypes = [i, vp, vp, i, i, vp, vp, i, i, i]
         L.orc_mesh_mask.argtypes = [i, vp, ctypes.c_int64, i, vp, vp, i, i, i]
@@ -97,8 +97,9 @@ def lattice(lat):
     return dict(c=c[:, :q], w=w[:q], opp=opp[:q], qi=qi[:q])
 
 
-def step(lat, coll, store, compute, f0, f1, bc_mask, missing, bcs, omega, force=None, force_field=None):
-    """One fused step in place: reads f0 (receiving aux recovery writes), writes f1."""
+def step(lat, coll, store, compute, f0, f1, bc_mask, missing, bcs, omega, force=None, force_field=None, cparam=0.0):
+    """One fused step in place: reads f0 (receiving aux recovery writes), writes f1.  coll: 0 BGK,
+    1 KBC, 2 Smagorinsky BGK with coefficient cparam."""
     nx, ny, nz = _dims(f0)
     bm = None if bc_mask is None else np.ascontiguousarray(bc_mask, dtype=np.uint8)
     mm = None if missing is None else np.ascontiguousarray(missing, dtype=np.uint8)
@@ -111,7 +112,7 @@ def step(lat, coll, store, compute, f0, f1, bc_mask, missing, bcs, omega, force=
     if bm is None:
         bm = np.zeros((1, nx, ny, nz), dtype=np.uint8)
     rc = lib().orc_step(lat, coll, store, compute, _p(f0), _p(f1), _p(bm), _p(mm), arr, len(bcs), mode, _p(fu), _p(ff),
-                        nx, ny, nz, float(omega))
+                        nx, ny, nz, float(omega), float(cparam))
     assert rc == 0
 
 
@@ -144,7 +145,7 @@ def stream(lat, store, fin):
     return fout
 
 
-def collide(lat, coll, store, compute, f, feq, rho, u, omega, force=None, force_field=None):
+def collide(lat, coll, store, compute, f, feq, rho, u, omega, force=None, force_field=None, cparam=0.0):
     fout = np.empty_like(f)
     nx, ny, nz = _dims(f)
     mode, fu, ff = 0, None, None
@@ -155,7 +156,7 @@ def collide(lat, coll, store, compute, f, feq, rho, u, omega, force=None, force_
     r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
     uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
     assert lib().orc_collide(lat, coll, store, compute, _p(np.ascontiguousarray(f)), _p(np.ascontiguousarray(feq)), _p(fout),
-                             _p(r), _p(uu), float(omega), mode, _p(fu), _p(ff), nx, ny, nz) == 0
+                             _p(r), _p(uu), float(omega), float(cparam), mode, _p(fu), _p(ff), nx, ny, nz) == 0
     return fout
 
 

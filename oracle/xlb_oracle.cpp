@@ -220,6 +220,7 @@ struct Ctx {
   int nx, ny, nz;
   size_t N;
   C omega;
+  C cparam = 0;  // Smagorinsky coefficient
   int force_mode;
   C force_u[3];
   const double* force_field;  // (d, N)
@@ -339,15 +340,31 @@ void kbc_shear(const Lattice& L, const C* fneq, C* s) {
   }
 }
 
-// Collision: 0 BGK (bgk.py:30-34), 1 KBC (kbc.py:268-296); then exact-difference forcing
+// Collision: 0 BGK (bgk.py:30-34), 1 KBC (kbc.py:268-296), 2 BGK with the Smagorinsky eddy
+// relaxation time (smagorinsky_les_bgk.py:73-89, coefficient cs); then exact-difference forcing
 // (forced_collision.py:41-47, exact_difference_force.py:71-78).
 template <class C>
-void collision(const Lattice& L, int coll, const C* f, const C* feq, C rho, const C* u, C omega, C* out) {
+void collision(const Lattice& L, int coll, const C* f, const C* feq, C rho, const C* u, C omega, C* out,
+               C cs = (C)0) {
   if (coll == 0) {
     for (int l = 0; l < L.q; ++l) {
       const C fneq = f[l] - feq[l];
       out[l] = f[l] - omega * fneq;
     }
+    return;
+  }
+  if (coll == 2) {
+    // "strain": fneq^2 over directions whose signed component sum is 1, 2 fneq^2 where it is >= 2
+    C strain = 0;
+    for (int l = 0; l < L.q; ++l) {
+      const int csum = L.c[0][l] + L.c[1][l] + (L.d == 3 ? L.c[2][l] : 0);
+      const C fneq = f[l] - feq[l];
+      if (csum == 1) strain += fneq * fneq;
+      if (csum >= 2) strain += (C)2.0 * fneq * fneq;
+    }
+    const C tau0 = (C)1.0 / omega;
+    const C tau = tau0 + (C)0.5 * (std::sqrt(tau0 * tau0 + (C)36.0 * (cs * cs) * std::sqrt(strain)) - tau0);
+    for (int l = 0; l < L.q; ++l) out[l] = f[l] - ((C)1.0 / tau) * (f[l] - feq[l]);
     return;
   }
   C fneq[27], shear[27], ds[27], dh[27];
@@ -570,7 +587,7 @@ void step_grid(Ctx<FMT, C>& X, int coll) {
         C rho, u[3], feq[27];
         macroscopic(L, fps, rho, u);
         equilibrium(L, rho, u, feq);
-        collision(L, coll, fps, feq, rho, u, X.omega, fpc);
+        collision(L, coll, fps, feq, rho, u, X.omega, fpc, X.cparam);
         if (X.force_mode != 0) {
           C F[3] = {X.force_u[0], X.force_u[1], X.force_u[2]};
           if (X.force_mode == 2)
@@ -644,7 +661,7 @@ int orc_set_threads(int n) {
 // One fused step.  f0 is read (and receives aux recovery writes), f1 is written.
 int orc_step(int lat, int coll, int store, int compute, void* f0, void* f1, const uint8_t* bc_mask,
              const uint8_t* missing, const OrcBc* bcs, int nbc, int force_mode, const double* force_u,
-             const double* force_field, int nx, int ny, int nz, double omega) {
+             const double* force_field, int nx, int ny, int nz, double omega, double cparam) {
   return dispatch(store, compute, [&](auto S, auto c) -> int {
     constexpr int FMT = decltype(S)::value;
     using C = decltype(c);
@@ -661,6 +678,7 @@ int orc_step(int lat, int coll, int store, int compute, void* f0, void* f1, cons
     X.nz = nz;
     X.N = (size_t)nx * ny * nz;
     X.omega = (C)omega;
+    X.cparam = (C)cparam;
     X.force_mode = force_mode;
     for (int d = 0; d < 3; ++d) X.force_u[d] = force_u ? (C)force_u[d] : (C)0;
     X.force_field = force_field;
@@ -755,8 +773,8 @@ int orc_stream(int lat, int store, const void* fin, void* fout, int nx, int ny, 
 // Standalone collision: fout = collision(f, feq, rho, u) (+ force) with rho/u read from fields
 // (doubles); BGK ignores them unless forced.
 int orc_collide(int lat, int coll, int store, int compute, const void* f, const void* feq, void* fout,
-                const double* rho, const double* u, double omega, int force_mode, const double* force_u,
-                const double* force_field, int nx, int ny, int nz) {
+                const double* rho, const double* u, double omega, double cparam, int force_mode,
+                const double* force_u, const double* force_field, int nx, int ny, int nz) {
   return dispatch(store, compute, [&](auto S, auto c) -> int {
     constexpr int FMT = decltype(S)::value;
     using C = decltype(c);
@@ -774,7 +792,7 @@ int orc_collide(int lat, int coll, int store, int compute, const void* f, const 
       C r = rho ? (C)rho[cc] : (C)0;
       if (u)
         for (int d = 0; d < L.d; ++d) uu[d] = (C)u[(size_t)d * X.N + cc];
-      collision(L, coll, fl, fe, r, uu, (C)omega, out);
+      collision(L, coll, fl, fe, r, uu, (C)omega, out, (C)cparam);
       if (force_mode != 0) {
         C F[3] = {0, 0, 0};
         for (int d = 0; d < L.d; ++d)

@@ -1,5 +1,5 @@
 """Writes the golden fixtures of the fused step: small seeded configurations covering every
-boundary kind, both collisions and the body force, run for 5 FP64 steps by the CPU oracle
+boundary kind, every collision and the body force, run for 5 FP64 steps by the CPU oracle
 (oracle/xlb_oracle.cpp, the restatement of nse_stepper.py:245-392).
 
 Each fixture holds the inputs (initial populations, bc_mask, missing mask, per-cell prescribed
@@ -7,7 +7,8 @@ values, the BC table, omega, force) and the expected populations after 5 steps. 
 built through the oracle's restatement of indices_boundary_masker.py:103-224 from the boundary
 indices of the product's BC classes.
 
-Usage: python tests/golden/make_golden.py   (writes tests/golden/step_*.npz)
+Usage: python tests/golden/make_golden.py [name ...]   (writes tests/golden/step_<name>.npz, all
+fixtures when no name is given)
 """
 
 import json
@@ -34,7 +35,11 @@ GOLDEN = {
     "d2q9_channel_zouhe_pressure": ("D2Q9", (16, 8), "channel2d_zouhe_pressure", "BGK", 1.1, None),
     "d3q19_do_nothing_forced": ("D3Q19", (8, 6, 6), "do_nothing", "BGK", 1.3, "uniform"),
     "d2q9_periodic_force_field": ("D2Q9", (12, 10), "periodic", "BGK", 1.3, "field"),
+    "d3q27_sphere_smagorinsky": ("D3Q27", (10, 8, 8), "sphere3d", "SmagorinskyLESBGK", 1.95, None),
 }
+
+COLLISION_ID = {"BGK": 0, "KBC": 1, "SmagorinskyLESBGK": 2}
+SMAGORINSKY_COEF = 0.17  # SmagorinskyLESBGK's default (smagorinsky_les_bgk.py:24)
 
 
 def build(name):
@@ -66,10 +71,12 @@ def build(name):
             sel = bc_mask.reshape(-1) == bc.id
             aux[sel] = vals[sel]
     out = dict(
-        lattice=np.array(lat), collision=np.array(1 if coll == "KBC" else 0), omega=np.array(omega),
+        lattice=np.array(lat), collision=np.array(COLLISION_ID[coll]), omega=np.array(omega),
         f_init=f0, bc_mask=bc_mask, missing=missing, aux=aux,
         bcs=np.array(json.dumps([oracle_bc_desc(bc) for bc in bcs])),
     )
+    if coll == "SmagorinskyLESBGK":
+        out["cparam"] = np.array(SMAGORINSKY_COEF)
     if force == "uniform":
         out["force"] = np.array([1e-4, -2e-4, 5e-5])[: vs.d]
     if force == "field":
@@ -91,9 +98,10 @@ def run_oracle(fx, steps=STEPS):
             orc.aux_init(orc.F64, f1, bc_mask, d["id"], aux)
     force = fx["force"] if "force" in fx else None
     field = fx["force_field"] if "force_field" in fx else None
+    cparam = float(fx["cparam"]) if "cparam" in fx else 0.0
     for _ in range(steps):
         orc.step(int(fx["lattice"]), int(fx["collision"]), orc.F64, 1, f0, f1, bc_mask, fx["missing"], bcs,
-                 float(fx["omega"]), force=force, force_field=field)
+                 float(fx["omega"]), force=force, force_field=field, cparam=cparam)
         f0, f1 = f1, f0
     return f0
 
@@ -102,8 +110,8 @@ def path(name):
     return os.path.join(HERE, f"step_{name}.npz")
 
 
-def main():
-    for name in GOLDEN:
+def main(names):
+    for name in names or GOLDEN:
         fx = build(name)
         fx["f_final"] = run_oracle(fx)
         np.savez_compressed(path(name), **fx)
@@ -111,4 +119,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

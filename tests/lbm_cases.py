@@ -222,6 +222,9 @@ CASES = {
     "sphere3d": ("D3Q27", (40, 24, 24), "KBC", 1.9, "rest"),
     "sphere3d_bgk": ("D3Q19", (40, 24, 24), "BGK", 1.6, "rest"),
     "do_nothing": ("D3Q19", (12, 10, 10), "BGK", 1.0, "random"),
+    "cavity_smagorinsky": ("D3Q19", (16, 16, 16), "SmagorinskyLESBGK", 1.95, "random"),
+    "channel2d_smagorinsky": ("D2Q9", (48, 24), "SmagorinskyLESBGK", 1.9, "rest"),
+    "sphere3d_smagorinsky": ("D3Q27", (40, 24, 24), "SmagorinskyLESBGK", 1.97, "rest"),
 }
 
 
@@ -230,7 +233,10 @@ LARGE_CASES = {
     # C1: 64^3 lid-driven cavity, halfway walls, lid u = (0.05, 0, 0), Re = 100 (nu = 0.05*64/100)
     "c1_cavity64": ("D3Q19", (64, 64, 64), "BGK", 1.0 / (3 * 0.05 * 64 / 100 + 0.5), "rest"),
 }
-BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway"}
+BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "cavity_smagorinsky": "cavity_halfway",
+          "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d"}
+
+ORACLE_COLLISION = {"BGK": 0, "KBC": 1, "SmagorinskyLESBGK": 2}
 
 
 def random_init(vs, shape, seed=0, du=0.02):
@@ -271,6 +277,8 @@ class Pair:
             self.grid, self.bcs, collision_type=coll, force_vector=force_vector, force_field=force_field
         )
         self.f0, self.f1, self.bc_mask, self.missing = self.stepper.prepare_fields()
+        self.o_coll = ORACLE_COLLISION[coll]
+        self.cparam = float(self.stepper.collision.collision_param)
         self.fmt, self.comp = policy_codes(policy)
         self.lat = vs.lattice_id
         self.shape3 = shape3
@@ -302,10 +310,9 @@ class Pair:
             self.f0, self.f1 = self.f1, self.f0
 
     def step_oracle(self, n):
-        coll = 1 if self.coll == "KBC" else 0
         for _ in range(n):
-            orc.step(self.lat, coll, self.fmt, self.comp, self.o_f0, self.o_f1, self.o_mask, self.o_missing, self.o_bcs,
-                     self.omega, force=self.force, force_field=self.force_field_np)
+            orc.step(self.lat, self.o_coll, self.fmt, self.comp, self.o_f0, self.o_f1, self.o_mask, self.o_missing,
+                     self.o_bcs, self.omega, force=self.force, force_field=self.force_field_np, cparam=self.cparam)
             self.o_f0, self.o_f1 = self.o_f1, self.o_f0
 
     def oracle_reference_f(self, steps, policy):
@@ -318,10 +325,9 @@ class Pair:
         for bc in self.bcs:
             if bc.needs_aux_init:
                 orc.aux_init(fmt, f1, self.o_mask, bc.id, oracle_aux(bc, self.o_mask, self.vs.d, policy))
-        coll = 1 if self.coll == "KBC" else 0
         for _ in range(steps):
-            orc.step(self.lat, coll, fmt, comp, f0, f1, self.o_mask, self.o_missing, self.o_bcs, self.omega,
-                     force=self.force, force_field=self.force_field_np)
+            orc.step(self.lat, self.o_coll, fmt, comp, f0, f1, self.o_mask, self.o_missing, self.o_bcs, self.omega,
+                     force=self.force, force_field=self.force_field_np, cparam=self.cparam)
             f0, f1 = f1, f0
         return decode(f0, fmt, self.vs.w)
 

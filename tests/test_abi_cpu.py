@@ -42,16 +42,16 @@ def test_error_channel_and_validation_without_gpu():
     # in-place step is rejected
     buf = ctypes.c_void_p(0x1000)
     rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, buf, buf, None, None, None, None,
-                     _lib.FORCE_NONE, None, None, 8, 8, 8, 1.0, 0, None)
+                     _lib.FORCE_NONE, None, None, 8, 8, 8, 1.0, 0.0, 0, None)
     assert rc != 0 and b"in place" in L.xlbh_last_error()
     # 2-D lattice with nz > 1 is rejected
     rc = L.xlbh_step(_lib.D2Q9, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000),
-                     None, None, None, None, _lib.FORCE_NONE, None, None, 8, 8, 4, 1.0, 0, None)
+                     None, None, None, None, _lib.FORCE_NONE, None, None, 8, 8, 4, 1.0, 0.0, 0, None)
     assert rc != 0 and b"2-D" in L.xlbh_last_error()
     # KBC on D3Q19 is rejected (kbc.py:183-185)
     rc = L.xlbh_collide(_lib.D3Q19, _lib.KBC, _lib.F32, _lib.COMPUTE_F32, _lib.FIELD_F32, ctypes.c_void_p(8),
                         ctypes.c_void_p(16), ctypes.c_void_p(24), ctypes.c_void_p(32), ctypes.c_void_p(40), 1.0,
-                        _lib.FORCE_NONE, None, None, 4, 4, 4, None)
+                        0.0, _lib.FORCE_NONE, None, None, 4, 4, 4, None)
     assert rc != 0 and b"KBC" in L.xlbh_last_error()
     # unknown lattice
     rc = L.xlbh_stream(7, _lib.F32, ctypes.c_void_p(8), ctypes.c_void_p(16), 4, 4, 4, None)

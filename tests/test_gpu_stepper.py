@@ -101,13 +101,13 @@ def test_c1_cavity64_1000_steps_matches_oracle(policy):
 
 
 @pytest.mark.parametrize("name", ["periodic", "cavity_fullway", "channel2d_zouhe", "channel2d_regularized", "sphere3d",
-                                  "periodic_field", "do_nothing"])
+                                  "periodic_field", "do_nothing", "cavity_smagorinsky", "sphere3d_smagorinsky"])
 def test_stepper_fp64_matches_oracle(name):
     compare(Pair(name, P.FP64FP64), 40)
 
 
 @pytest.mark.parametrize("policy", [P.FP32FP16, P.FP32BF16, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP64FP32])
-@pytest.mark.parametrize("name", ["periodic", "cavity_halfway", "sphere3d", "channel2d_zouhe"])
+@pytest.mark.parametrize("name", ["periodic", "cavity_halfway", "sphere3d", "channel2d_zouhe", "cavity_smagorinsky"])
 def test_stepper_reduced_storage_matches_oracle(name, policy):
     compare(Pair(name, policy), 20)
 

@@ -374,6 +374,48 @@ def test_kbc_relaxes_shear_like_bgk(lat):
     np.testing.assert_allclose(orc.collide(lat, 1, orc.F64, 1, feq, feq, rho, u, 1.6), feq, atol=1e-15)
 
 
+@pytest.mark.parametrize("lat", [0, 1, 2])
+def test_smagorinsky_pinned(lat):
+    """SmagorinskyLESBGK (smagorinsky_les_bgk.py:73-89) pinned by what its definition fixes:
+    * Cs = 0 is BGK with omega, and fneq = 0 leaves feq unchanged for any Cs;
+    * a single-direction perturbation fneq = eps e_l relaxes by 1/tau with
+      tau (tau - tau0) = 9 Cs^2 sqrt(S), the quadratic the reference's
+      tau = tau0 + (sqrt(tau0^2 + 36 Cs^2 sqrt(S)) - tau0) / 2 solves, where S is eps^2 when the
+      signed component sum of c_l is 1, 2 eps^2 when it is >= 2 and 0 otherwise (tau = tau0);
+    * the other directions, whose fneq is zero, stay at feq."""
+    d, q = LATS[lat]
+    L = orc.lattice(lat)
+    c = L["c"]
+    shp = grid_shape(lat, 4)
+    rng = np.random.default_rng(5)
+    rho = 1 + 0.05 * rng.standard_normal(shp)
+    u = 0.05 * rng.standard_normal((d,) + shp)
+    feq = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, rho, u, feq)
+    f = feq + 1e-3 * rng.standard_normal(feq.shape)
+    for omega in (0.7, 1.9):
+        bgk = orc.collide(lat, 0, orc.F64, 1, f, feq, None, None, omega)
+        np.testing.assert_allclose(orc.collide(lat, 2, orc.F64, 1, f, feq, None, None, omega, cparam=0.0), bgk,
+                                   rtol=0, atol=1e-15)
+        np.testing.assert_allclose(orc.collide(lat, 2, orc.F64, 1, feq, feq, None, None, omega, cparam=0.3), feq,
+                                   rtol=0, atol=1e-15)
+    cs, omega, eps = 0.17, 1.8, 2e-3
+    tau0 = 1.0 / omega
+    seen = set()
+    for l in range(q):
+        fl = feq.copy()
+        fl[l] += eps
+        out = orc.collide(lat, 2, orc.F64, 1, fl, feq, None, None, omega, cparam=cs)
+        tau = eps / (fl[l] - out[l])
+        csum = int(c[:, l].sum())
+        strain = eps * eps if csum == 1 else (2 * eps * eps if csum >= 2 else 0.0)
+        seen.add(min(max(csum, 0), 2))
+        np.testing.assert_allclose(tau * (tau - tau0), 9 * cs * cs * np.sqrt(strain), rtol=1e-9, atol=1e-14)
+        others = np.delete(np.arange(q), l)
+        np.testing.assert_allclose(out[others], feq[others], rtol=0, atol=1e-15)
+    assert seen == {0, 1, 2}
+
+
 def test_momentum_exchange_symmetric_body_has_no_lift():
     lat, (d, q) = 1, LATS[1]
     n = 16

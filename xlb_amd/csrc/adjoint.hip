@@ -786,7 +786,7 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
   const bool recompute = (flags & XLBH_ADJ_RECOMPUTE) != 0;
   for (int j = 1; recompute && j < n_steps; ++j) {
     if (int e = xlbh_step(lattice, XLBH_BGK, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
-                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega,
+                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega, 0.0,
                           timestep0 + j - 1, stream))
       return e;
   }

@@ -35,7 +35,8 @@ static int validate_step(int lattice, int collision, int store, int compute, con
                          int force_mode, const double* force_uniform, const void* force_field, int nx, int ny,
                          int nz) {
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
-  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC, "unknown collision id");
+  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC || collision == XLBH_SMAGORINSKY_BGK,
+               "unknown collision id");
   XLBH_REQUIRE(valid_store(store), "unknown store format");
   XLBH_REQUIRE(compute == XLBH_COMPUTE_F32 || compute == XLBH_COMPUTE_F64, "unknown compute type");
   XLBH_REQUIRE(f_a && f_b, "population fields must be non-null");
@@ -55,26 +56,26 @@ extern "C" {
 
 const char* xlbh_last_error(void) { return g_last_error.c_str(); }
 
-int xlbh_abi_version(void) { return 1; }
+int xlbh_abi_version(void) { return 2; }
 
 int xlbh_step(int lattice, int collision, int store, int compute, const void* f_in, void* f_out,
               const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
               int force_mode, const double* force_uniform, const void* force_field, int nx, int ny, int nz,
-              double omega, int timestep, void* stream) {
+              double omega, double collision_param, int timestep, void* stream) {
   (void)timestep;
   clear_error();
   if (int e = validate_step(lattice, collision, store, compute, f_in, f_out, bc_mask, missing_bits, bc_table,
                             force_mode, force_uniform, force_field, nx, ny, nz))
     return e;
   StepCall c{collision, store, compute, f_in, f_out, bc_mask, missing_bits, aux, bc_table,
-             force_mode, force_uniform, force_field, nx, ny, nz, omega, (hipStream_t)stream};
+             force_mode, force_uniform, force_field, nx, ny, nz, omega, collision_param, (hipStream_t)stream};
   return dispatch_step(lattice, c);
 }
 
 int xlbh_run(int lattice, int collision, int store, int compute, void* f_a, void* f_b, const uint8_t* bc_mask,
              const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table, int force_mode,
              const double* force_uniform, const void* force_field, int nx, int ny, int nz, double omega,
-             int timestep0, int n_steps, void* stream) {
+             double collision_param, int timestep0, int n_steps, void* stream) {
   (void)timestep0;
   clear_error();
   if (int e = validate_step(lattice, collision, store, compute, f_a, f_b, bc_mask, missing_bits, bc_table,
@@ -82,7 +83,7 @@ int xlbh_run(int lattice, int collision, int store, int compute, void* f_a, void
     return e;
   XLBH_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
   StepCall c{collision, store, compute, f_a, f_b, bc_mask, missing_bits, aux, bc_table,
-             force_mode, force_uniform, force_field, nx, ny, nz, omega, (hipStream_t)stream};
+             force_mode, force_uniform, force_field, nx, ny, nz, omega, collision_param, (hipStream_t)stream};
   for (int t = 0; t < n_steps; ++t) {
     c.f_in = (t % 2 == 0) ? f_a : f_b;
     c.f_out = (t % 2 == 0) ? f_b : f_a;
@@ -94,7 +95,7 @@ int xlbh_run(int lattice, int collision, int store, int compute, void* f_a, void
 int xlbh_record(int lattice, int collision, int store, int compute, const void* f_start, void* states,
                 const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
                 int force_mode, const double* force_uniform, const void* force_field, int nx, int ny, int nz,
-                double omega, int timestep0, int n_states, void* stream) {
+                double omega, double collision_param, int timestep0, int n_states, void* stream) {
   clear_error();
   XLBH_REQUIRE(n_states >= 0, "n_states must be >= 0");
   if (n_states == 0) return 0;
@@ -105,7 +106,7 @@ int xlbh_record(int lattice, int collision, int store, int compute, const void* 
     return e;
   const size_t field_bytes = (size_t)lattice_q(lattice) * (size_t)nx * ny * nz * store_bytes(store);
   StepCall c{collision, store, compute, f_start, states, bc_mask, missing_bits, aux, bc_table,
-             force_mode, force_uniform, force_field, nx, ny, nz, omega, (hipStream_t)stream};
+             force_mode, force_uniform, force_field, nx, ny, nz, omega, collision_param, (hipStream_t)stream};
   for (int j = 0; j < n_states; ++j) {
     c.f_in = j == 0 ? f_start : static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes;
     c.f_out = static_cast<char*>(states) + (size_t)j * field_bytes;

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_collide(const typename Fmt<FMT>::raw* _
                                                  const typename Fmt<FMT>::raw* __restrict__ feq,
                                                  typename Fmt<FMT>::raw* __restrict__ fout,
                                                  const FR* __restrict__ rho, const FR* __restrict__ u,
-                                                 C omega, int force_mode, double fu0, double fu1,
+                                                 C omega, C cparam, int force_mode, double fu0, double fu1,
                                                  double fu2, const C* __restrict__ force_field, Sweep sw,
                                                  uint32_t N) {
   constexpr LatTables T = make_tables(LAT);
@@ -128,6 +128,21 @@ __global__ __launch_bounds__(256) void k_collide(const typename Fmt<FMT>::raw* _
   if constexpr (COLL == XLBH_BGK) {
 #pragma unroll
     for (int l = 0; l < Q; ++l) out[l] = ff[l] - omega * (ff[l] - fe[l]);
+  } else if constexpr (COLL == XLBH_SMAGORINSKY_BGK) {
+    // smagorinsky_les_bgk.py:40-94 with feq as given (the fused step's branch in physics.hpp)
+    C strain = C(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const int cs = T.c[0][l] + T.c[1][l] + T.c[2][l];
+      const C fneq = ff[l] - fe[l];
+      if (cs == 1) strain += fneq * fneq;
+      if (cs >= 2) strain += C(2) * fneq * fneq;
+    }
+    const C tau0 = C(1) / omega;
+    const C tau = tau0 + C(0.5) * (sqrt(tau0 * tau0 + C(36) * (cparam * cparam) * sqrt(strain)) - tau0);
+    const C inv_tau = C(1) / tau;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) out[l] = ff[l] - inv_tau * (ff[l] - fe[l]);
   } else {
     C fneq[Q];
 #pragma unroll
@@ -362,18 +377,20 @@ int xlbh_moments(int lattice, int store, int compute, int field, int which, cons
 }
 
 int xlbh_collide(int lattice, int collision, int store, int compute, int field, const void* f,
-                 const void* feq, void* fout, const void* rho, const void* u, double omega, int force_mode,
+                 const void* feq, void* fout, const void* rho, const void* u, double omega,
+                 double collision_param, int force_mode,
                  const double* force_uniform, const void* force_field, int nx, int ny, int nz, void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(f && feq && fout, "collide needs f, feq and fout");
-  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC, "unknown collision");
+  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC || collision == XLBH_SMAGORINSKY_BGK,
+               "unknown collision");
   XLBH_REQUIRE(!(collision == XLBH_KBC && lattice == XLBH_D3Q19),
                "KBC collision is defined for D2Q9 and D3Q27 only (kbc.py:183-185)");
   XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
   XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
   XLBH_REQUIRE(force_mode != XLBH_FORCE_FIELD || force_field, "force field is null");
-  XLBH_REQUIRE((collision == XLBH_BGK && force_mode == XLBH_FORCE_NONE) || (rho && u),
+  XLBH_REQUIRE((collision != XLBH_KBC && force_mode == XLBH_FORCE_NONE) || (rho && u),
                "KBC and forced collisions need rho and u");
   const Sweep sw = make_sweep(nx, ny, nz);
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
@@ -389,13 +406,18 @@ int xlbh_collide(int lattice, int collision, int store, int compute, int field, 
       using R = typename Fmt<FMT>::raw;
       return with_field(field, [&](auto fr) -> int {
         using FR = decltype(fr);
+        const C cp = (C)collision_param;
         if (collision == XLBH_BGK)
           hipLaunchKernelGGL((k_collide<LAT, XLBH_BGK, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
-                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega,
+                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega, cp,
+                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
+        else if (collision == XLBH_SMAGORINSKY_BGK)
+          hipLaunchKernelGGL((k_collide<LAT, XLBH_SMAGORINSKY_BGK, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
+                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega, cp,
                              force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
         else if constexpr (LAT != XLBH_D3Q19)
           hipLaunchKernelGGL((k_collide<LAT, XLBH_KBC, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
-                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega,
+                             (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega, cp,
                              force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
         XLBH_LAUNCH_CHECK();
         return 0;

@@ -106,7 +106,7 @@ __device__ __forceinline__ C kbc_shear(int l, const C* pi) {
 //   KBC: f - beta (2 ds + gamma dh)                               kbc.py:268-296
 //   force: f += feq(rho, u + F) - feq(rho, u)                     exact_difference_force.py:71-86
 template <int LAT, int COLL, class C>
-__device__ __forceinline__ void collide(C* f, C omega, bool has_force, const C* F) {
+__device__ __forceinline__ void collide(C* f, C omega, bool has_force, const C* F, C cparam = C(0)) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   C rho, u[3];
@@ -117,6 +117,27 @@ __device__ __forceinline__ void collide(C* f, C omega, bool has_force, const C* 
     for (int l = 0; l < Q; ++l) {
       const C fe = feq_l<LAT>(l, rho, u, usqr);
       f[l] = f[l] - omega * (f[l] - fe);
+    }
+  } else if constexpr (COLL == XLBH_SMAGORINSKY_BGK) {
+    // BGK with the Smagorinsky eddy relaxation time (smagorinsky_les_bgk.py:78-94): "strain"
+    // sums fneq^2 over the directions whose component sum is 1 and 2 fneq^2 over those whose
+    // component sum is >= 2, as the reference does; tau = tau0 + (sqrt(tau0^2 + 36 Cs^2
+    // sqrt(strain)) - tau0) / 2.
+    C strain = C(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const int cs = T.c[0][l] + T.c[1][l] + T.c[2][l];
+      const C fneq = f[l] - feq_l<LAT>(l, rho, u, usqr);
+      if (cs == 1) strain += fneq * fneq;
+      if (cs >= 2) strain += C(2) * fneq * fneq;
+    }
+    const C tau0 = C(1) / omega;
+    const C tau = tau0 + C(0.5) * (sqrt(tau0 * tau0 + C(36) * (cparam * cparam) * sqrt(strain)) - tau0);
+    const C inv_tau = C(1) / tau;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const C fe = feq_l<LAT>(l, rho, u, usqr);
+      f[l] = f[l] - inv_tau * (f[l] - fe);
     }
   } else {
     C pi[6];

@@ -32,6 +32,7 @@ struct StepParams {
   double fu[3];
   int force_mode;
   double omega;
+  double cparam;  // Smagorinsky constant (XLBH_SMAGORINSKY_BGK)
   Sweep sw;
   uint32_t N;
 };
@@ -131,7 +132,7 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) F[dd] = ff[(size_t)dd * p.N + cell];
     }
-    collide<LAT, COLL>(f, (C)p.omega, has_force, F);
+    collide<LAT, COLL>(f, (C)p.omega, has_force, F, (C)p.cparam);
   }
 
   if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
@@ -353,6 +354,7 @@ struct StepCall {
   const void* force_field;
   int nx, ny, nz;
   double omega;
+  double cparam;  // Smagorinsky constant (XLBH_SMAGORINSKY_BGK)
   hipStream_t stream;
 };
 
@@ -369,6 +371,7 @@ int launch_step_t(const StepCall& c) {
   p.force_mode = c.force_mode;
   for (int d = 0; d < 3; ++d) p.fu[d] = (c.force_mode == XLBH_FORCE_UNIFORM) ? c.force_uniform[d] : 0.0;
   p.omega = c.omega;
+  p.cparam = c.cparam;
   p.sw = make_sweep(c.nx, c.ny, c.nz);
   p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
   const bool wide = (uint64_t)p.N * sizeof(typename Fmt<FMT>::raw) > 0xffffffffull;
@@ -419,6 +422,8 @@ int launch_step_lat(const StepCall& c) {
   }
   if (c.collision == XLBH_BGK) {
     XLBH_STEP_CASES(XLBH_BGK)
+  } else if (c.collision == XLBH_SMAGORINSKY_BGK) {
+    XLBH_STEP_CASES(XLBH_SMAGORINSKY_BGK)
   } else if constexpr (has_kbc) {
     if (c.collision == XLBH_KBC) {
       XLBH_STEP_CASES(XLBH_KBC)

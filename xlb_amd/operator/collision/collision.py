@@ -9,7 +9,8 @@ from xlb_amd.operator.operator import Operator, expect_field, field_extents
 from xlb_amd.precision_policy import Precision
 
 
-def _collide(op, collision, f, feq, fout, rho, u, omega, force_mode=_lib.FORCE_NONE, force_vector=None, force_field=None):
+def _collide(op, collision, f, feq, fout, rho, u, omega, force_mode=_lib.FORCE_NONE, force_vector=None, force_field=None,
+             collision_param=0.0):
     _lib.require_cuda(f, feq, fout, rho, u, force_field)
     nx, ny, nz = field_extents(f)
     vs = op.velocity_set
@@ -35,7 +36,7 @@ def _collide(op, collision, f, feq, fout, rho, u, omega, force_mode=_lib.FORCE_N
         _lib.lib().xlbh_collide(
             op.lattice_id, collision, op.format_of(f), op.compute_id, field_id,
             _lib.ptr(f), _lib.ptr(feq), _lib.ptr(fout), _lib.ptr(rho), _lib.ptr(u), float(omega),
-            force_mode, fu_ptr, _lib.ptr(force_field), nx, ny, nz, _lib.stream_of(f),
+            float(collision_param), force_mode, fu_ptr, _lib.ptr(force_field), nx, ny, nz, _lib.stream_of(f),
         ),
         type(op).__name__,
     )
@@ -47,6 +48,7 @@ class Collision(Operator):
     """Base class of collision operators."""
 
     collision_id = None
+    collision_param = 0.0
 
 
 class BGK(Collision):
@@ -76,10 +78,24 @@ class KBC(Collision):
 
 
 class SmagorinskyLESBGK(Collision):
-    """Not provided by this backend (no in-scope configuration uses it)."""
+    """BGK with a Smagorinsky eddy relaxation time (smagorinsky_les_bgk.py:14-94):
+    tau = tau0 + (sqrt(tau0^2 + 36 Cs^2 sqrt(strain)) - tau0) / 2 with the reference's "strain"
+    (fneq^2 over the directions whose component sum is 1, 2 fneq^2 where it is >= 2)."""
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError("SmagorinskyLESBGK is not implemented by the MI355X backend")
+    collision_id = _lib.SMAGORINSKY_BGK
+
+    def __init__(self, velocity_set=None, precision_policy=None, compute_backend=None, smagorinsky_coef: float = 0.17):
+        self.smagorinsky_coef = smagorinsky_coef
+        super().__init__(velocity_set=velocity_set, precision_policy=precision_policy, compute_backend=compute_backend)
+
+    @property
+    def collision_param(self):
+        return float(self.smagorinsky_coef)
+
+    @Operator.register_backend(ComputeBackend.MI355X)
+    def mi355x_implementation(self, f, feq, fout, rho, u, omega):
+        return _collide(self, _lib.SMAGORINSKY_BGK, f, feq, fout, None, None, omega,
+                        collision_param=self.smagorinsky_coef)
 
 
 class ForcedCollision(Collision):
@@ -103,6 +119,7 @@ class ForcedCollision(Collision):
         self.force_vector = force_vector
         self.force_field = force_field
         self.collision_id = collision_operator.collision_id
+        self.collision_param = getattr(collision_operator, "collision_param", 0.0)
 
     @property
     def force_mode(self):
@@ -111,5 +128,6 @@ class ForcedCollision(Collision):
     @Operator.register_backend(ComputeBackend.MI355X)
     def mi355x_implementation(self, f, feq, fout, rho, u, omega):
         return _collide(
-            self, self.collision_id, f, feq, fout, rho, u, omega, self.force_mode, self.force_vector, self.force_field
+            self, self.collision_id, f, feq, fout, rho, u, omega, self.force_mode, self.force_vector, self.force_field,
+            self.collision_param,
         )

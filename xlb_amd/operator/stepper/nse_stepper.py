@@ -185,7 +185,10 @@ class IncompressibleNavierStokesStepper(Stepper):
         return _lib.FORCE_NONE, None, None
 
     def _collision_id(self):
-        return _lib.KBC if self.collision_type == "KBC" else _lib.BGK
+        return {"KBC": _lib.KBC, "SmagorinskyLESBGK": _lib.SMAGORINSKY_BGK}.get(self.collision_type, _lib.BGK)
+
+    def _collision_param(self):
+        return float(getattr(self.collision, "collision_param", 0.0))
 
     def _launch_args(self, f_0, f_1, bc_mask, missing_mask):
         nx, ny, nz = field_extents(f_0)
@@ -215,7 +218,7 @@ class IncompressibleNavierStokesStepper(Stepper):
             _lib.lib().xlbh_step(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
-                nx, ny, nz, float(omega), int(timestep), _lib.stream_of(f_0),
+                nx, ny, nz, float(omega), self._collision_param(), int(timestep), _lib.stream_of(f_0),
             ),
             "IncompressibleNavierStokesStepper",
         )
@@ -284,7 +287,8 @@ class IncompressibleNavierStokesStepper(Stepper):
             _lib.lib().xlbh_record(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_start), _lib.ptr(states), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
-                nx, ny, nz, float(omega), int(timestep), int(num_states), _lib.stream_of(f_start),
+                nx, ny, nz, float(omega), self._collision_param(), int(timestep), int(num_states),
+                _lib.stream_of(f_start),
             ),
             "IncompressibleNavierStokesStepper.record",
         )
@@ -351,7 +355,8 @@ class IncompressibleNavierStokesStepper(Stepper):
             _lib.lib().xlbh_run(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
-                nx, ny, nz, float(omega), int(timestep), int(num_steps), _lib.stream_of(f_0),
+                nx, ny, nz, float(omega), self._collision_param(), int(timestep), int(num_steps),
+                _lib.stream_of(f_0),
             ),
             "IncompressibleNavierStokesStepper.run",
         )

@@ -63,9 +63,11 @@ def periodic_box_3d(n, policy, omega=1.0, seed=0):
     return Workload(st, f0, f1, bm, mm, omega)
 
 
-def lid_driven_cavity_3d(n, policy, lid_u=0.02, omega=1.0, walls="fullway"):
+def lid_driven_cavity_3d(n, policy, lid_u=0.02, omega=1.0, walls="fullway", collision="BGK"):
     """mlups_3d.py:45-90: EquilibriumBC lid (top face without edges), bounce-back on the other
-    five full faces, rest start.  walls = "fullway" (as mlups_3d.py) or "halfway"."""
+    five full faces, rest start.  walls = "fullway" (as mlups_3d.py) or "halfway"; collision =
+    "BGK" (mlups_3d.py) or "SmagorinskyLESBGK" (the LES cavity of
+    examples/out_of_core/lid_driven_cavity.py: lid 0.06, tau 0.501)."""
     BC = xa.operator.boundary_condition
     vs, grid = _setup("D3Q19", policy, (n, n, n))
     box = grid.bounding_box_indices()
@@ -73,7 +75,7 @@ def lid_driven_cavity_3d(n, policy, lid_u=0.02, omega=1.0, walls="fullway"):
     wall_idx = _faces(box, ["bottom", "left", "right", "front", "back"], 3)
     wall = BC.FullwayBounceBackBC(indices=wall_idx) if walls == "fullway" else BC.HalfwayBounceBackBC(indices=wall_idx)
     bcs = [BC.EquilibriumBC(rho=1.0, u=(lid_u, 0.0, 0.0), indices=lid), wall]
-    st = xa.operator.stepper.IncompressibleNavierStokesStepper(grid, bcs, collision_type="BGK")
+    st = xa.operator.stepper.IncompressibleNavierStokesStepper(grid, bcs, collision_type=collision)
     f0, f1, bm, mm = st.prepare_fields()
     return Workload(st, f0, f1, bm, mm, omega)
 
