@@ -16,7 +16,8 @@ Also reported on the same line:
 * adjoint -- backward (checkpoint recompute + adjoint kernels) over forward time per step for the
   differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50) and the 256^3 cavity;
 * kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, with the momentum-exchange drag
-  every 100 steps inside the timed loop;
+  every 100 steps inside the timed loop; grads_sphere -- the same with a GradsApproximationBC
+  sphere;
 * mesh_masker -- surface voxelisation of an 81,920-triangle sphere mesh into the same grid;
 * les_cavity -- the LES lid-driven cavity of examples/out_of_core/lid_driven_cavity.py (D3Q19
   SmagorinskyLESBGK, Cs = 0.17, 256^3, lid u = 0.06, tau = 0.501, FP32), with the Q-criterion
@@ -268,13 +269,14 @@ def adjoint_leg(xa, torch, quick=False):
     return res
 
 
-def kbc_sphere_leg(xa, torch, steps=500, quick=False):
-    """D3Q27 KBC wind tunnel with the drag every 100 steps inside the timed loop."""
+def kbc_sphere_leg(xa, torch, steps=500, quick=False, sphere_bc="halfway"):
+    """D3Q27 KBC wind tunnel with the drag every 100 steps inside the timed loop; the sphere is a
+    halfway bounce-back or a Grad's-approximation wall."""
     from xlb_amd.workloads import wind_tunnel_sphere
 
     shape = (128, 64, 64) if quick else (256, 128, 128)
     radius, center = (6, (32, 32, 32)) if quick else (12, (64, 64, 64))
-    wl = wind_tunnel_sphere(shape, xa.PrecisionPolicy.FP32FP32, radius, center)
+    wl = wind_tunnel_sphere(shape, xa.PrecisionPolicy.FP32FP32, radius, center, sphere_bc=sphere_bc)
     st = wl.stepper
     mt = xa.operator.force.MomentumTransfer(wl.extra["sphere_bc"])
     f0, f1 = st.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, 100)
@@ -292,12 +294,13 @@ def kbc_sphere_leg(xa, torch, steps=500, quick=False):
     cells = int(np.prod(shape))
     bpl = 2 * 27 * 4
     mlups = cells * steps / wall / 1e6
-    res = dict(workload=f"D3Q27 KBC wind tunnel {shape[0]}x{shape[1]}x{shape[2]}, sphere r={radius}, Re=200, "
-                        f"{steps} steps with momentum-exchange drag every 100, FP32",
+    wall_name = "halfway bounce-back" if sphere_bc == "halfway" else "GradsApproximationBC"
+    res = dict(workload=f"D3Q27 KBC wind tunnel {shape[0]}x{shape[1]}x{shape[2]}, {wall_name} sphere r={radius}, "
+                        f"Re=200, {steps} steps with momentum-exchange drag every 100, FP32",
                mlups=mlups, achieved_gbs=mlups * 1e6 * bpl / 1e9, frac=mlups * 1e6 * bpl / 1e9 / HBM_PEAK_GBS,
                drag_last=[float(v) for v in drags[-1].cpu()], finite=bool(torch.isfinite(f0).all().item()))
     res["cd_last"] = 2.0 * res["drag_last"][0] / (0.04**2 * np.pi * radius**2)
-    log(f"bench: kbc sphere {mlups:.0f} MLUPS ({res['achieved_gbs']:.0f} GB/s), drag {res['drag_last']}")
+    log(f"bench: kbc sphere ({sphere_bc}) {mlups:.0f} MLUPS ({res['achieved_gbs']:.0f} GB/s), drag {res['drag_last']}")
     del wl, st, f0, f1
     torch.cuda.empty_cache()
     return res
@@ -597,6 +600,7 @@ def main():
         extras["precision_headline"] = headline_precision_leg(xa, torch, 128 if args.quick else n, 1000)
         extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
         extras["kbc_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick)
+        extras["grads_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick, sphere_bc="grads")
         extras["mesh_masker"] = mesh_leg(xa, torch, subdivisions=4 if args.quick else 6)
         extras["les_cavity"] = les_cavity_leg(xa, torch, 128 if args.quick else n, args.steps, args.warmup)
     cpu = None

@@ -61,7 +61,8 @@ enum {
   XLBH_BC_ZOUHE_PRESSURE = 6,         /* bc_zouhe.py:314-338                (streaming step) */
   XLBH_BC_REGULARIZED_VELOCITY = 7,   /* bc_regularized.py:134-168          (streaming step) */
   XLBH_BC_REGULARIZED_PRESSURE = 8,   /* bc_regularized.py:170-196          (streaming step) */
-  XLBH_BC_EXTRAPOLATION_OUTFLOW = 9   /* bc_extrapolation_outflow.py:152-195 (streaming + post-collision) */
+  XLBH_BC_EXTRAPOLATION_OUTFLOW = 9,  /* bc_extrapolation_outflow.py:152-195 (streaming + post-collision) */
+  XLBH_BC_GRADS_APPROXIMATION = 10    /* bc_grads_approximation.py:181-210  (streaming step, 3-D) */
 };
 
 /* Body-force modes of the exact-difference forcing (xlb/operator/force/exact_difference_force.py). */

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 # kinds (same numbering as the C ABI's XLBH_BC_*)
 EQUILIBRIUM, HALFWAY_BB, FULLWAY_BB, DO_NOTHING = 1, 2, 3, 4
 ZOUHE_VELOCITY, ZOUHE_PRESSURE, REGULARIZED_VELOCITY, REGULARIZED_PRESSURE = 5, 6, 7, 8
-EXTRAPOLATION_OUTFLOW = 9
+EXTRAPOLATION_OUTFLOW, GRADS_APPROXIMATION = 9, 10
 
 # store formats
 F32, F64, F16, BF16, F16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3, 4, 5

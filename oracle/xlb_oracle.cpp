@@ -196,7 +196,8 @@ constexpr bool shifted(int fmt) { return fmt == 4 || fmt == 5; }
 // ------------------------------------------------------------------------------------------
 struct OrcBc {
   int32_t id;
-  int32_t kind;  // 1 equilibrium, 2 halfway, 3 fullway, 4 do-nothing, 5/6 zou-he v/p, 7/8 regularized v/p, 9 extrapolation
+  int32_t kind;  // 1 equilibrium, 2 halfway, 3 fullway, 4 do-nothing, 5/6 zou-he v/p, 7/8 regularized v/p,
+                 // 9 extrapolation, 10 Grad's approximation
   double rho;
   double u[3];
 };
@@ -492,6 +493,18 @@ void bc_functional(const Ctx<FMT, C>& X, const OrcBc& bc, int x, int y, int z, c
       equilibrium(L, rho, u, feq);
       bounceback_nonequilibrium(L, out, feq, mm);
       if (bc.kind == 8) regularize_fpop(L, out, feq);
+      break;
+    }
+    case 10: {  // GradsApproximationBC, functional_method1 (bc_grads_approximation.py:181-210)
+      const C one = (C)1.0;
+      const C weight = (C)0.5;
+      for (int l = 0; l < L.q; ++l)
+        if (mm[l] == 1)
+          out[l] = ((one - weight) * out[L.opp[l]] + weight * (f_pre[l] + f_pre[L.opp[l]])) / (one + weight);
+      C rho, u[3], feq[27];
+      macroscopic(L, out, rho, u);
+      equilibrium(L, rho, u, feq);
+      regularize_fpop(L, out, feq);  // regularize_fpop of bc_grads_approximation.py:112-135
       break;
     }
     default:

@@ -36,6 +36,8 @@ GOLDEN = {
     "d3q19_do_nothing_forced": ("D3Q19", (8, 6, 6), "do_nothing", "BGK", 1.3, "uniform"),
     "d2q9_periodic_force_field": ("D2Q9", (12, 10), "periodic", "BGK", 1.3, "field"),
     "d3q27_sphere_smagorinsky": ("D3Q27", (10, 8, 8), "sphere3d", "SmagorinskyLESBGK", 1.95, None),
+    "d3q19_sphere_grads": ("D3Q19", (12, 10, 10), "sphere3d_grads", "BGK", 1.7, None),
+    "d3q19_cavity_grads": ("D3Q19", (8, 8, 8), "cavity_grads", "BGK", 1.4, None),
 }
 
 COLLISION_ID = {"BGK": 0, "KBC": 1, "SmagorinskyLESBGK": 2}

@@ -204,6 +204,19 @@ def make_bcs(name, grid, xa):
         ]
     if name == "do_nothing":
         return [BC.DoNothingBC(indices=ne["right"]), BC.HalfwayBounceBackBC(indices=ne["left"])]
+    if name == "cavity_grads":  # lid-driven cavity with Grad's-approximation walls
+        walls = unique_faces(box, ["bottom", "left", "right", "front", "back"], 3)
+        return [BC.GradsApproximationBC(indices=walls), BC.EquilibriumBC(rho=1.0, u=(0.05, 0.0, 0.0), indices=ne["top"])]
+    if name == "sphere3d_grads":  # wind tunnel with a Grad's-approximation sphere (interior indices, padded)
+        walls = unique_faces(box, ["bottom", "top", "front", "back"], 3)
+        r = max(2, shape[1] // 6)
+        sph = sphere_indices(shape, (shape[0] // 4, shape[1] // 2, shape[2] // 2), r)
+        return [
+            BC.FullwayBounceBackBC(indices=walls),
+            BC.RegularizedBC("velocity", prescribed_value=(0.04, 0.0, 0.0), indices=ne["left"]),
+            BC.ExtrapolationOutflowBC(indices=ne["right"]),
+            BC.GradsApproximationBC(indices=sph),
+        ]
     raise KeyError(name)
 
 
@@ -225,6 +238,9 @@ CASES = {
     "cavity_smagorinsky": ("D3Q19", (16, 16, 16), "SmagorinskyLESBGK", 1.95, "random"),
     "channel2d_smagorinsky": ("D2Q9", (48, 24), "SmagorinskyLESBGK", 1.9, "rest"),
     "sphere3d_smagorinsky": ("D3Q27", (40, 24, 24), "SmagorinskyLESBGK", 1.97, "rest"),
+    "cavity_grads": ("D3Q19", (16, 16, 16), "BGK", 1.6, "random"),
+    "sphere3d_grads": ("D3Q27", (40, 24, 24), "KBC", 1.9, "rest"),
+    "sphere3d_grads_bgk": ("D3Q19", (40, 24, 24), "BGK", 1.7, "rest"),
 }
 
 
@@ -234,7 +250,8 @@ LARGE_CASES = {
     "c1_cavity64": ("D3Q19", (64, 64, 64), "BGK", 1.0 / (3 * 0.05 * 64 / 100 + 0.5), "rest"),
 }
 BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "cavity_smagorinsky": "cavity_halfway",
-          "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d"}
+          "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d",
+          "sphere3d_grads_bgk": "sphere3d_grads"}
 
 ORACLE_COLLISION = {"BGK": 0, "KBC": 1, "SmagorinskyLESBGK": 2}
 

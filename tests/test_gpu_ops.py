@@ -268,8 +268,12 @@ def test_fullway_bc_known_answer(lat):
         assert np.all(f[i][~inside] == 2.0)
 
 
-@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
-@pytest.mark.parametrize("which", ["halfway", "do_nothing", "zouhe_v", "zouhe_p", "reg_v", "reg_p", "extrap"])
+STANDALONE_BC = [(lat, which) for which in ["halfway", "do_nothing", "zouhe_v", "zouhe_p", "reg_v", "reg_p", "extrap"]
+                 for lat in ["D2Q9", "D3Q19", "D3Q27"]]
+STANDALONE_BC += [("D3Q19", "grads"), ("D3Q27", "grads")]  # GradsApproximationBC is 3-D only
+
+
+@pytest.mark.parametrize("lat,which", STANDALONE_BC)
 def test_standalone_bc_matches_oracle(lat, which):
     vs = setup(lat, P.FP64FP64)
     n = 14
@@ -287,6 +291,7 @@ def test_standalone_bc_matches_oracle(lat, which):
             "reg_v": lambda: BC.RegularizedBC("velocity", profile=lambda idx: 0.01 + 0.001 * idx[1], indices=side),
             "reg_p": lambda: BC.RegularizedBC("pressure", prescribed_value=0.99, indices=side),
             "extrap": lambda: BC.ExtrapolationOutflowBC(indices=ne["right"]),
+            "grads": lambda: BC.GradsApproximationBC(indices=sphere(n, vs.d)),
         }[which]()
 
     g, bcs, bc_mask, missing = _mask_case(vs, n, lambda g: [make(g)])
@@ -305,12 +310,12 @@ def test_standalone_bc_matches_oracle(lat, which):
     np.testing.assert_allclose(out, expect, rtol=1e-13, atol=1e-16)
 
 
-@pytest.mark.parametrize("lat", ["D3Q19", "D3Q27"])
+@pytest.mark.parametrize("case", ["sphere3d", "sphere3d_bgk", "sphere3d_grads_bgk"])
 @pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
-def test_momentum_transfer_matches_oracle(lat, policy):
+def test_momentum_transfer_matches_oracle(case, policy):
     from lbm_cases import Pair
 
-    pair = Pair("sphere3d" if lat == "D3Q27" else "sphere3d_bgk", policy)
+    pair = Pair(case, policy)
     pair.step_product(30)
     pair.step_oracle(30)
     sph = pair.bcs[-1]

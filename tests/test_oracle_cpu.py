@@ -523,6 +523,56 @@ def test_regularized_reproduces_nonequilibrium_stress(lat):
     np.testing.assert_allclose(reg.sum(0)[sel], rho[sel], rtol=1e-13)
 
 
+@pytest.mark.parametrize("lat", [1, 2])
+def test_grads_bc_pinned(lat):
+    """GradsApproximationBC (bc_grads_approximation.py:181-210, wall weight 1/2) pinned by what
+    fixes it:
+    * the rest equilibrium w_l is a fixed point for any missing set;
+    * with weight 1/2, ((1 - q) a + q (b + c)) / (1 + q) is the plain mean of the three sources, so
+      each missing l first becomes mean(f_post[opp l], f_pre[l], f_pre[opp l]), in ascending l and in
+      place (a later missing direction sees an earlier one's new value; the random mask has cells
+      with both l and opp l missing);
+    * the result carries exactly that intermediate's density, momentum and second moment (fourth-
+      order isotropy of the regularisation), and lies in the span of the ten Hermite vectors
+      w_l {1, c_l, c_l c_l - I/3}, which together fix it;
+    * cells with another id are untouched."""
+    d, q = LATS[lat]
+    c, w = ref_lattice(lat)
+    opp = orc.lattice(lat)["opp"]
+    shp = (6, 5, 4)
+    rng = np.random.default_rng(21)
+    bc_mask = (rng.random((1,) + shp) < 0.5).astype(np.uint8)
+    miss = (rng.random((q,) + shp) < 0.3).astype(np.uint8)
+    miss[0] = 0
+    both = miss.astype(bool) & miss[opp].astype(bool)
+    assert both[:, bc_mask[0] == 1].any()
+    wv = w.reshape(-1, 1, 1, 1)
+    rest = np.broadcast_to(wv, (q,) + shp).copy()
+    desc = dict(id=1, kind=orc.GRADS_APPROXIMATION)
+    np.testing.assert_allclose(orc.apply_bc(lat, orc.F64, 1, desc, rest, rest, bc_mask, miss), rest, rtol=0,
+                               atol=1e-15)
+    fpre = wv * (1 + 0.1 * rng.standard_normal((q,) + shp))
+    fpost = wv * (1 + 0.1 * rng.standard_normal((q,) + shp))
+    out = orc.apply_bc(lat, orc.F64, 1, desc, fpre, fpost, bc_mask, miss)
+    sel = bc_mask[0] == 1
+    np.testing.assert_array_equal(out[:, ~sel], fpost[:, ~sel])
+    ft = fpost.copy()
+    for l in range(q):
+        m = miss[l].astype(bool) & sel
+        ft[l][m] = ((ft[opp[l]] + fpre[l] + fpre[opp[l]]) / 3)[m]
+    pi = lambda g: np.einsum("al,bl,l...->ab...", c, c, g)
+    np.testing.assert_allclose(out.sum(0)[sel], ft.sum(0)[sel], rtol=1e-13)
+    np.testing.assert_allclose(np.tensordot(c, out, axes=(1, 0))[:, sel], np.tensordot(c, ft, axes=(1, 0))[:, sel],
+                               atol=1e-14)
+    np.testing.assert_allclose(pi(out)[..., sel], pi(ft)[..., sel], atol=1e-14)
+    basis = [np.ones(q)] + [c[a].astype(float) for a in range(3)]
+    basis += [c[a] * c[b] - (a == b) / 3.0 for a in range(3) for b in range(a, 3)]
+    B = (w[:, None] * np.array(basis).T)
+    cells = out[:, sel]
+    coef, *_ = np.linalg.lstsq(B, cells, rcond=None)
+    assert np.abs(B @ coef - cells).max() < 1e-14
+
+
 @pytest.mark.parametrize("lat", [0, 1, 2])
 def test_extrapolation_outflow_keeps_uniform_flow(lat):
     """A uniform equilibrium flow through an EquilibriumBC inlet and an ExtrapolationOutflowBC

@@ -29,6 +29,7 @@ BC_ZOUHE_PRESSURE = 6
 BC_REGULARIZED_VELOCITY = 7
 BC_REGULARIZED_PRESSURE = 8
 BC_EXTRAPOLATION_OUTFLOW = 9
+BC_GRADS_APPROXIMATION = 10
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 ADJ_RECOMPUTE, ADJ_BEGIN, ADJ_END = 1, 2, 4
 

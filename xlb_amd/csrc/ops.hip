@@ -237,6 +237,8 @@ __global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc
     const C pv = aux[cell];
     zouhe_cell<LAT>(f, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
                     kind >= XLBH_BC_REGULARIZED_VELOCITY);
+  } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
+    grads_cell<LAT>(f, mm, pre);
   }
 #pragma unroll
   for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fpost + (size_t)l * N + cell, f[l], T.w[l]);
@@ -432,9 +434,12 @@ int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id, cons
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(f_pre && f_post && bc_mask, "apply_bc needs f_pre, f_post and bc_mask");
-  XLBH_REQUIRE(kind >= XLBH_BC_NONE && kind <= XLBH_BC_EXTRAPOLATION_OUTFLOW, "unknown BC kind");
+  XLBH_REQUIRE(kind >= XLBH_BC_NONE && kind <= XLBH_BC_GRADS_APPROXIMATION, "unknown BC kind");
+  XLBH_REQUIRE(kind != XLBH_BC_GRADS_APPROXIMATION || lattice != XLBH_D2Q9,
+               "GradsApproximationBC is 3-D only (bc_grads_approximation.py:75-77)");
   XLBH_REQUIRE(bc_id >= 0 && bc_id <= 255, "bc_id out of range");
   const bool needs_missing = kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW ||
+                             kind == XLBH_BC_GRADS_APPROXIMATION ||
                              (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE);
   XLBH_REQUIRE(!needs_missing || missing_bits, "this BC kind needs the missing bits");
   XLBH_REQUIRE(!(kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) || aux,

@@ -215,6 +215,47 @@ __device__ __forceinline__ C bc_fsum(const C* f, uint32_t mm) {
   return known + middle;
 }
 
+// Regularisation of f (in place) around the equilibrium of its own moments:
+// f_l = feq_l(rho, u) + 4.5 w_l Q_l : Pi_neq(f), with rho, u taken from f.  The projection of
+// bc_grads_approximation.py:112-135 (regularize_fpop), applied after the moments are recomputed
+// (:203-208).
+template <int LAT, class C>
+__device__ __forceinline__ void regularize_own_moments(C* f) {
+  constexpr LatTables T = make_tables(LAT);
+  C rho, u[3];
+  macroscopic<LAT>(f, rho, u);
+  const C usqr = usqr15<LAT>(u);
+  C pi[6];
+  pi_neq<LAT>(f, rho, u, usqr, pi);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    C qp = C(0);
+#pragma unroll
+    for (int t = 0; t < T.nt; ++t) qp += C(T.qi[l][t]) * pi[t];
+    f[l] = feq_l<LAT>(l, rho, u, usqr) + C(4.5) * C(T.w[l]) * qp;
+  }
+}
+
+// Grad's-approximation wall (bc_grads_approximation.py:181-210, the reference's active
+// functional_method1): every missing l gets the interpolated bounce-back with the fixed wall
+// weight q = 1/2, ((1 - q) f_post[opp l] + q (f_pre[l] + f_pre[opp l])) / (1 + q), in place and in
+// ascending l (a later missing direction sees an earlier one's new value), then the populations
+// are regularised around their own moments.  `pre(l)` returns the cell's pre-stream population l.
+template <int LAT, class C, class Pre>
+__device__ __forceinline__ void grads_cell(C* f, uint32_t mm, Pre&& pre) {
+  constexpr LatTables T = make_tables(LAT);
+  const C q = C(0.5);
+  const C one = C(1);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    if (bit(mm, l)) {
+      const int o = T.opp[l];
+      f[l] = ((one - q) * f[o] + q * (pre(l) + pre(o))) / (one + q);
+    }
+  }
+  regularize_own_moments<LAT>(f);
+}
+
 // Zou-He / regularized functional on post-stream populations f (in place), with the
 // prescribed value `pv` (normal speed or density).  bc_zouhe.py:279-338,
 // bc_regularized.py:134-196, helper_functions_bc.py:88-122.

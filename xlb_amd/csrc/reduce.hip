@@ -86,6 +86,8 @@ __global__ __launch_bounds__(256) void k_momentum(const typename Fmt<FMT>::raw* 
       } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
         zouhe_cell<LAT>(fps, mm, aux[cell], kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
                         kind >= XLBH_BC_REGULARIZED_VELOCITY);
+      } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
+        grads_cell<LAT>(fps, mm, [&](int l) { return fpc[l]; });
       }
 #pragma unroll
       for (int d = 0; d < T.d; ++d) {

@@ -109,6 +109,8 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
       const bool vel = (kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY);
       const bool reg = (kind >= XLBH_BC_REGULARIZED_VELOCITY);
       zouhe_cell<LAT>(f, mm, pv, vel, reg);
+    } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
+      grads_cell<LAT>(f, mm, [&](int l) { return ld_pop<FMT, C>(fin.at(l, cell), T.w[l]); });
     }
   }
 

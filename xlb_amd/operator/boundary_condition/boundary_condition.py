@@ -308,7 +308,20 @@ class ExtrapolationOutflowBC(BoundaryCondition):
 
 
 class GradsApproximationBC(BoundaryCondition):
-    """Not provided by this backend."""
+    """Grad's-approximation wall (bc_grads_approximation.py:28-321; streaming step, 3-D): the
+    missing populations take the interpolated bounce-back with the wall weight 1/2 the reference
+    uses for index-defined walls (:195-201), then every population is regularised around the
+    cell's own density and velocity (:203-208, :112-135).  Index-defined walls need padding; a mesh
+    definition needs a mesh-distance masker, which the reference does not provide either."""
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError("GradsApproximationBC is not implemented by the MI355X backend")
+    kind = _lib.BC_GRADS_APPROXIMATION
+
+    def __init__(self, velocity_set=None, precision_policy=None, compute_backend=None, indices=None, mesh_vertices=None):
+        self.u = (0, 0, 0)
+        super().__init__(ImplementationStep.STREAMING, velocity_set, precision_policy, compute_backend, indices, mesh_vertices)
+        self.needs_mesh_distance = True
+        if self.mesh_vertices is None:
+            assert self.indices is not None
+            self.needs_padding = True
+        if self.velocity_set.d == 2:
+            raise NotImplementedError("This BC is not implemented in 2D!")

@@ -230,6 +230,13 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.BC_ZOUHE_PRESSURE, _lib.BC_REGULARIZED_VELOCITY, _lib.BC_REGULARIZED_PRESSURE,
     )
 
+    def _check_adjoint_supported(self):
+        if self.collision_type != "BGK":
+            raise NotImplementedError("the adjoint step is implemented for BGK collision")
+        for bc in self.boundary_conditions:
+            if bc.kind == _lib.BC_GRADS_APPROXIMATION:
+                raise NotImplementedError("the adjoint step does not support GradsApproximationBC")
+
     def _adjoint_cells(self):
         """(pointer, count) of the cells the adjoint's boundary pass runs over: every cell whose BC
         kind has contributions outside the pull (bounce-back, extrapolation, do-nothing, Zou-He /
@@ -249,8 +256,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         """Reverse mode of one step (xlbh_adjoint_step): given the state f_t a forward step read and
         lam_out = dL/df_{t+1}, write lam_in = dL/df_t and add dL/dg to ``grad_force`` ((d, nx, ny, nz)
         float64) when the stepper has a body force.  BGK only."""
-        if self.collision_type != "BGK":
-            raise NotImplementedError("the adjoint step is implemented for BGK collision")
+        self._check_adjoint_supported()
         nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_t, None, bc_mask, missing_mask)
         q, ext = self.velocity_set.q, (nx, ny, nz)
         expect_field(lam_out, q, ext, "lam_out", self.compute_dtype)
@@ -310,8 +316,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         ``recompute=False`` the states are taken as recorded by ``record``.  A ``workspace`` (from
         ``adjoint_workspace``) selects the fused one-kernel reverse step; a sweep over several
         segments shares it, with ``begin`` on its first call and ``end`` on its last.  BGK only."""
-        if self.collision_type != "BGK":
-            raise NotImplementedError("the adjoint step is implemented for BGK collision")
+        self._check_adjoint_supported()
         if self._has_solid:
             raise NotImplementedError("the adjoint step does not support solid (255) cells")
         nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_ckpt, None, bc_mask, missing_mask)

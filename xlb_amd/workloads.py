@@ -80,10 +80,12 @@ def lid_driven_cavity_3d(n, policy, lid_u=0.02, omega=1.0, walls="fullway", coll
     return Workload(st, f0, f1, bm, mm, omega)
 
 
-def wind_tunnel_sphere(shape, policy, radius, center, u_in=0.04, reynolds=200.0, lattice="D3Q27", collision="KBC"):
-    """flow_past_sphere_3d.py:41-125: sphere as HalfwayBounceBack cells, RegularizedBC velocity
-    inlet at x = 0, ExtrapolationOutflowBC at x = nx-1, FullwayBounceBack on the y/z faces;
-    omega from the Reynolds number on the diameter."""
+def wind_tunnel_sphere(shape, policy, radius, center, u_in=0.04, reynolds=200.0, lattice="D3Q27", collision="KBC",
+                       sphere_bc="halfway"):
+    """flow_past_sphere_3d.py:41-125: sphere as HalfwayBounceBack cells (sphere_bc="halfway") or
+    GradsApproximationBC cells ("grads"), RegularizedBC velocity inlet at x = 0,
+    ExtrapolationOutflowBC at x = nx-1, FullwayBounceBack on the y/z faces; omega from the Reynolds
+    number on the diameter."""
     BC = xa.operator.boundary_condition
     vs, grid = _setup(lattice, policy, shape)
     box = grid.bounding_box_indices()
@@ -96,7 +98,7 @@ def wind_tunnel_sphere(shape, policy, radius, center, u_in=0.04, reynolds=200.0,
         BC.FullwayBounceBackBC(indices=walls),
         BC.RegularizedBC("velocity", prescribed_value=(u_in, 0.0, 0.0), indices=ne["left"]),
         BC.ExtrapolationOutflowBC(indices=ne["right"]),
-        BC.HalfwayBounceBackBC(indices=sphere),
+        BC.HalfwayBounceBackBC(indices=sphere) if sphere_bc == "halfway" else BC.GradsApproximationBC(indices=sphere),
     ]
     nu = u_in * 2 * radius / reynolds
     omega = 1.0 / (3.0 * nu + 0.5)
