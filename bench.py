@@ -562,6 +562,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="smaller extra legs")
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--only", default=None,
+                    help="comma-separated extra legs to run alone (kbc_sphere, grads_sphere, les_cavity, mesh_masker, "
+                         "adjoint, periodic), printing their JSON; for profiling")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.size)
@@ -576,6 +579,21 @@ def main():
     n = args.size
     if args.profile:
         profile_pass(xa, torch, n, args.steps)
+        return
+    if args.only:
+        legs = {
+            "kbc_sphere": lambda: kbc_sphere_leg(xa, torch, quick=args.quick),
+            "grads_sphere": lambda: kbc_sphere_leg(xa, torch, quick=args.quick, sphere_bc="grads"),
+            "les_cavity": lambda: les_cavity_leg(xa, torch, 128 if args.quick else n, args.steps, args.warmup),
+            "mesh_masker": lambda: mesh_leg(xa, torch, subdivisions=4 if args.quick else 6),
+            "adjoint": lambda: adjoint_leg(xa, torch, quick=args.quick),
+            "periodic": lambda: periodic_leg(xa, torch, n, args.steps, args.warmup),
+        }
+        names = args.only.split(",")
+        unknown = [k for k in names if k not in legs]
+        if unknown:
+            sys.exit(f"unknown legs {unknown}; choose from {sorted(legs)}")
+        print(json.dumps({k: legs[k]() for k in names}))
         return
     log(f"bench: fp32 leg, {n}^3, warmup {args.warmup}, steps {args.steps}")
     fp32 = headline_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n, args.steps, args.warmup)
