@@ -26,3 +26,4 @@ import xlb_amd.operator.postprocess
 import xlb_amd.operator.stepper
 import xlb_amd.helper
 import xlb_amd.autodiff
+import xlb_amd.utils
