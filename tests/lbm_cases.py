@@ -204,6 +204,14 @@ def make_bcs(name, grid, xa):
         ]
     if name == "do_nothing":
         return [BC.DoNothingBC(indices=ne["right"]), BC.HalfwayBounceBackBC(indices=ne["left"])]
+    if name == "channel2d_zouhe_interior":  # a Zou-He inlet on an interior column (padded: missing[0] set)
+        walls = unique_faces(box, ["bottom", "top"], 2)
+        col = [[2] * (shape[1] - 2), list(range(1, shape[1] - 1))]
+        return [
+            BC.HalfwayBounceBackBC(indices=walls),
+            BC.ZouHeBC("velocity", prescribed_value=(0.03, 0.0), indices=col),
+            BC.ExtrapolationOutflowBC(indices=ne["right"]),
+        ]
     if name == "cavity_grads":  # lid-driven cavity with Grad's-approximation walls
         walls = unique_faces(box, ["bottom", "left", "right", "front", "back"], 3)
         return [BC.GradsApproximationBC(indices=walls), BC.EquilibriumBC(rho=1.0, u=(0.05, 0.0, 0.0), indices=ne["top"])]

@@ -32,6 +32,9 @@ ADJ_CASES = {
     "tiny2d_cavity": ("D2Q9", (7, 6), "cavity2d", 1.5, "uniform"),
     "tiny3d_sphere": ("D3Q19", (8, 6, 6), "sphere3d", 1.6, "field"),
     "tiny3d_do_nothing": ("D3Q27", (5, 4, 4), "do_nothing", 1.3, "field"),
+    "tiny3d_grads_sphere": ("D3Q19", (8, 6, 6), "sphere3d_grads", 1.6, "field"),
+    "tiny3d_grads_cavity": ("D3Q19", (5, 5, 4), "cavity_grads", 1.4, None),
+    "tiny2d_zouhe_interior": ("D2Q9", (8, 6), "channel2d_zouhe_interior", 1.2, "field"),
     "periodic2d": ("D2Q9", (24, 20), "periodic", 1.3, "field"),
     "periodic3d_uniform": ("D3Q19", (10, 8, 6), "periodic", 1.6, "uniform"),
     "channel2d_zouhe": ("D2Q9", (32, 16), "channel2d_zouhe", 1.2, "field"),
@@ -41,6 +44,8 @@ ADJ_CASES = {
     "sphere3d": ("D3Q19", (24, 16, 16), "sphere3d", 1.6, "field"),
     "sphere3d_q27": ("D3Q27", (20, 12, 12), "sphere3d", 1.6, "field"),
     "do_nothing": ("D3Q19", (12, 10, 10), "do_nothing", 1.0, "field"),
+    "grads_sphere": ("D3Q19", (24, 16, 16), "sphere3d_grads", 1.6, "field"),
+    "grads_sphere_q27": ("D3Q27", (20, 12, 12), "sphere3d_grads", 1.6, "field"),
 }
 
 TINY = [k for k in ADJ_CASES if k.startswith("tiny")]
@@ -221,7 +226,7 @@ def test_adjoint_rejects_bad_arguments():
         kbc.adjoint_step(c.f0, lam, torch.zeros_like(lam), c.bc_mask, c.missing, c.omega)
 
 
-@pytest.mark.parametrize("name", ["tiny2d_channel", "tiny2d_regularized", "sphere3d", "do_nothing"])
+@pytest.mark.parametrize("name", ["tiny2d_channel", "tiny2d_regularized", "sphere3d", "do_nothing", "grads_sphere"])
 def test_boundary_pass_list_equals_grid_scan(name):
     """The boundary pass over the compacted cell list (what the stepper passes) and over the whole
     grid (bc_cells = NULL through the C ABI) give the same adjoint."""
@@ -247,11 +252,13 @@ def test_boundary_pass_list_equals_grid_scan(name):
 
 
 @pytest.mark.parametrize("name", ["channel2d_zouhe", "channel2d_regularized", "tiny2d_zouhe_pressure", "sphere3d",
-                                  "sphere3d_q27", "do_nothing", "cavity_fullway"])
+                                  "sphere3d_q27", "do_nothing", "cavity_fullway", "grads_sphere", "grads_sphere_q27",
+                                  "tiny3d_grads_cavity", "tiny2d_zouhe_interior"])
 def test_fused_and_split_reverse_agree(name):
     """The fused one-kernel reverse step (boundary contributions carried in the workspace, Zou-He /
     regularized Jacobian by the hand-written one-pass adjoint) against the two-kernel reverse step
-    (atomics, Jacobian by forward-mode dual numbers), over a checkpointed rollout in FP64."""
+    (atomics, Zou-He Jacobian by forward-mode dual numbers; Grad's by the same one-pass adjoint),
+    over a checkpointed rollout in FP64."""
     c = Case(name, P.FP64FP64, seed=6)
     f_fused, g_fused = c.gradients(9, every=4, fused=True)
     f_split, g_split = c.gradients(9, every=4, fused=False)

@@ -198,16 +198,11 @@ def test_periodic_full_size_conserves_mass():
     np.testing.assert_allclose(m1[1:], m0[1:], atol=1e-9)
 
 
-def test_grads_bc_unsupported_paths_raise():
+def test_grads_bc_on_a_mesh_is_rejected():
     """GradsApproximationBC on a mesh needs a mesh-distance masker (mesh_boundary_masker.py:216,
-    as in the reference), and the adjoint does not support it: both raise instead of running."""
-    from lbm_cases import Pair
+    as in the reference): it raises instead of running."""
     from mesh_cases import box_mesh
 
-    pair = Pair("cavity_grads", P.FP32FP32)
-    lam = torch.zeros_like(pair.f0, dtype=torch.float32)
-    with pytest.raises(NotImplementedError, match="GradsApproximationBC"):
-        pair.stepper.adjoint_step(pair.f0, lam, lam.clone(), pair.bc_mask, pair.missing, pair.omega)
     BC = xa.operator.boundary_condition
     grid = xa.grid.grid_factory((12, 12, 12), compute_backend=xa.ComputeBackend.MI355X)
     bc = BC.GradsApproximationBC(mesh_vertices=box_mesh((3.5, 3.5, 3.5), (7.5, 7.5, 7.5)).reshape(-1, 3))

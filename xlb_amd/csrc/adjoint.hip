@@ -161,12 +161,14 @@ __device__ __forceinline__ void zouhe_cell_adjoint(const C* f, uint32_t mm, C pv
 #pragma unroll
   for (int l = Q - 1; l >= 0; --l) {
     if (bit(mm, l)) {
+      // zero before adding: for l = opp(l) = 0 (missing[0] at padded interior cells) the slot
+      // reads itself
       const int o = T.opp[l];
       const C g = Gf[l];
+      Gf[l] = C(0);
       Gf[o] += g;
       Gfeq[l] += g;
       Gfeq[o] -= g;
-      Gf[l] = C(0);
     }
   }
   C A, B[3];
@@ -184,6 +186,71 @@ __device__ __forceinline__ void zouhe_cell_adjoint(const C* f, uint32_t mm, C pv
   for (int l = 0; l < Q; ++l) {
     if (bit(mm, T.opp[l])) Gf[l] += C(2) * gfsum;
     else if (!bit(mm, l)) Gf[l] += gfsum;
+  }
+}
+
+// Reverse mode of grads_cell (physics.hpp): given G = dL/dh for h = grads_cell(fps, pre), returns
+// Gf = dL/dfps and Gpre = dL/dpre (the cell's own pre-stream populations).  Forward: every missing l
+// (ascending, in place) takes (f_o + pre_l + pre_o) / 3 (weight 1/2), then h_l = feq_l(rho, u) +
+// 4.5 w_l sum_t qi_lt Pi_t with rho, u, Pi_t = sum_m cc_mt (f_m - feq_m) of the interpolated f.
+// Reverse: P_t = 4.5 sum_l G_l w_l qi_lt; dL/df_m = sum_t cc_mt P_t + A + B.(c_m - u) with A, B the
+// equilibrium moments (eq_moments) of G - cc P; then the interpolations undone in descending l.
+template <int LAT, class C>
+__device__ __forceinline__ void grads_cell_adjoint(const C* fps, const C* pre, uint32_t mm, const C* G, C* Gf,
+                                                   C* Gpre) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  const C third = C(1) / C(3);
+  C f[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    f[l] = fps[l];
+    Gpre[l] = C(0);
+  }
+  const C q = C(0.5);
+  const C one = C(1);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    if (bit(mm, l)) {
+      const int o = T.opp[l];
+      f[l] = ((one - q) * f[o] + q * (pre[l] + pre[o])) / (one + q);
+    }
+  }
+  C rho, u[3];
+  macroscopic<LAT>(f, rho, u);
+  C P[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+#pragma unroll
+  for (int l = 0; l < Q; ++l)
+#pragma unroll
+    for (int t = 0; t < T.nt; ++t) P[t] += G[l] * C(4.5) * C(T.w[l]) * C(T.qi[l][t]);
+  C lamq[Q];
+#pragma unroll
+  for (int m = 0; m < Q; ++m) {
+    C g = C(0);
+#pragma unroll
+    for (int t = 0; t < T.nt; ++t) g += C(T.cc[m][t]) * P[t];
+    Gf[m] = g;
+    lamq[m] = G[m] - g;
+  }
+  C A, B[3];
+  eq_moments<LAT>(lamq, u, u, A, B);
+#pragma unroll
+  for (int m = 0; m < Q; ++m) {
+    C s = A;
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) s += B[d] * (C(T.c[d][m]) - u[d]);
+    Gf[m] += s;
+  }
+#pragma unroll
+  for (int l = Q - 1; l >= 0; --l) {
+    if (bit(mm, l)) {
+      const int o = T.opp[l];
+      const C g = Gf[l] * third;
+      Gf[l] = C(0);  // before adding: l = opp(l) = 0 reads itself
+      Gf[o] += g;
+      Gpre[l] += g;
+      Gpre[o] += g;
+    }
   }
 }
 
@@ -248,6 +315,8 @@ __device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, i
     const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
     zouhe_cell<LAT>(h, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
                     kind >= XLBH_BC_REGULARIZED_VELOCITY);
+  } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
+    grads_cell<LAT>(h, mm, [&](int l) { return ld_pop<FMT, C>(fin + (size_t)l * p.N + cell, T.w[l]); });
   }
   if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
 #pragma unroll
@@ -273,6 +342,15 @@ __device__ __forceinline__ void cell_force(const AdjParams& p, uint32_t cell, C*
 
 __device__ __forceinline__ bool heavy_kind(int kind) {
   return kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE;
+}
+
+// The cell's own pre-stream populations (Grad's approximation reads them).
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ void own_pre(const AdjParams& p, uint32_t cell, C* pre) {
+  constexpr LatTables T = make_tables(LAT);
+  const typename Fmt<FMT>::raw* fin = static_cast<const typename Fmt<FMT>::raw*>(p.f_in);
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) pre[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + cell, T.w[l]);
 }
 
 // Kernel A: every cell pushes G_ps into lam_in (a permutation) and adds its force gradient.
@@ -304,8 +382,10 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
   } else if (kind == XLBH_BC_FULLWAY_BB) {
 #pragma unroll
     for (int k = 0; k < Q; ++k) G[k] = lam[T.opp[k]];
-  } else if (heavy_kind(kind) || kind == XLBH_BC_EQUILIBRIUM || kind == XLBH_BC_DO_NOTHING) {
-    // no dependence on the pulled values through this cell's pull (Zou-He is added by kernel B)
+  } else if (heavy_kind(kind) || kind == XLBH_BC_EQUILIBRIUM || kind == XLBH_BC_DO_NOTHING ||
+             kind == XLBH_BC_GRADS_APPROXIMATION) {
+    // no dependence on the pulled values through this cell's pull (Zou-He and Grad's are added by
+    // kernel B)
 #pragma unroll
     for (int k = 0; k < Q; ++k) G[k] = C(0);
     if (kind == XLBH_BC_EQUILIBRIUM && p.force_mode != XLBH_FORCE_NONE && p.grad_force) {
@@ -352,11 +432,12 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
   if (id == 0 || id == 255) return;
   const xlbh_bc_desc* d = p.table + id;
   const int kind = d->kind;
+  const bool grads = kind == XLBH_BC_GRADS_APPROXIMATION;
   if (kind != XLBH_BC_HALFWAY_BB && kind != XLBH_BC_EXTRAPOLATION_OUTFLOW && kind != XLBH_BC_DO_NOTHING &&
-      !heavy_kind(kind))
+      !heavy_kind(kind) && !grads)
     return;
   const uint32_t mm = p.missing[cell];
-  if (part > 0 && (!heavy_kind(kind) || bit(mm, part - 1))) return;
+  if (part > 0 && !heavy_kind(kind)) return;
   C fps[Q], h[Q], lam[Q];
   adj_prepare<LAT, FMT, C>(p, cell, x, y, z, kind, mm, d, fps, h, lam);
   C* li = static_cast<C*>(p.lam_in);
@@ -397,7 +478,19 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
 #pragma unroll
       for (int k = 0; k < Q; ++k) add(k, cell, G[k]);
     }
-    if (heavy_kind(kind) || kind == XLBH_BC_DO_NOTHING) {
+    if (grads) {
+      // the whole Grad's Jacobian in one reverse pass: pulled values and own pre-stream values
+      C pre[Q], Gf[Q], Gpre[Q];
+      own_pre<LAT, FMT, C>(p, cell, pre);
+      grads_cell_adjoint<LAT>(fps, pre, mm, G, Gf, Gpre);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+        add(k, ((uint32_t)sx * ny + sy) * nz + sz, Gf[k]);
+        add(k, cell, Gpre[k]);
+      }
+    }
+    if (heavy_kind(kind) || kind == XLBH_BC_DO_NOTHING || grads) {
       if (has_force && p.grad_force) {
 #pragma unroll
         for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
@@ -411,7 +504,8 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
     const bool reg = kind >= XLBH_BC_REGULARIZED_VELOCITY;
     const int k0 = part > 0 ? part - 1 : 0, k1 = part > 0 ? part : Q;
     for (int k = k0; k < k1; ++k) {
-      if (bit(mm, k)) continue;  // overwritten by the BC: no dependence
+      // every column, missing ones included: a missing slot's old value is still read when
+      // l = opp(l) = 0 is missing (padded interior cells) or when opp(l) is missing too
       Dual<C> hd[Q];
 #pragma unroll
       for (int l = 0; l < Q; ++l) hd[l] = Dual<C>(fps[l], l == k ? C(1) : C(0));
@@ -462,7 +556,8 @@ __global__ __launch_bounds__(256) void k_adjoint_bc_list(AdjParams p) {
 // is read and cleared by the flagged consumer cell in the next reverse step.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ bool own_kind(int kind) {
-  return kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW || kind == XLBH_BC_DO_NOTHING;
+  return kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW || kind == XLBH_BC_DO_NOTHING ||
+         kind == XLBH_BC_GRADS_APPROXIMATION;
 }
 
 template <int LAT, int FMT, class C>
@@ -543,6 +638,11 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
       const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
       zouhe_cell_adjoint<LAT>(fps, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
                               kind >= XLBH_BC_REGULARIZED_VELOCITY, Gh, G);
+    } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
+      // pulled values pushed as usual; own pre-stream values carried in d_own
+      C pre[Q];
+      own_pre<LAT, FMT, C>(p, cell, pre);
+      grads_cell_adjoint<LAT>(fps, pre, mm, Gh, G, dn);
     } else {
 #pragma unroll
       for (int k = 0; k < Q; ++k) G[k] = Gh[k];

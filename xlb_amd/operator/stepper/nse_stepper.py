@@ -228,19 +228,17 @@ class IncompressibleNavierStokesStepper(Stepper):
     _ADJOINT_BC_KINDS = (
         _lib.BC_HALFWAY_BB, _lib.BC_EXTRAPOLATION_OUTFLOW, _lib.BC_DO_NOTHING, _lib.BC_ZOUHE_VELOCITY,
         _lib.BC_ZOUHE_PRESSURE, _lib.BC_REGULARIZED_VELOCITY, _lib.BC_REGULARIZED_PRESSURE,
+        _lib.BC_GRADS_APPROXIMATION,
     )
 
     def _check_adjoint_supported(self):
         if self.collision_type != "BGK":
             raise NotImplementedError("the adjoint step is implemented for BGK collision")
-        for bc in self.boundary_conditions:
-            if bc.kind == _lib.BC_GRADS_APPROXIMATION:
-                raise NotImplementedError("the adjoint step does not support GradsApproximationBC")
 
     def _adjoint_cells(self):
         """(pointer, count) of the cells the adjoint's boundary pass runs over: every cell whose BC
         kind has contributions outside the pull (bounce-back, extrapolation, do-nothing, Zou-He /
-        regularized); fullway bounce-back and equilibrium cells have none."""
+        regularized, Grad's approximation); fullway bounce-back and equilibrium cells have none."""
         if not self._has_bc_cells:
             return None, 0
         if self._bc_cells is None:
