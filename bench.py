@@ -487,17 +487,18 @@ def load_traffic(n):
     return None
 
 
-def profile_pass(xa, torch, n, steps):
+def profile_pass(xa, torch, n, steps, policies=("FP32FP32",)):
     """What the rocprofv3 runs profile: one 1 GiB device copy (the counter calibration), then
-    ``steps`` headline steps."""
+    ``steps`` headline steps for each storage policy named."""
     a = torch.ones(COPY_CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
     del a, b
-    r = headline_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n, steps, 5, python_loop=False)
-    log(f"bench: profile pass {r['mlups']:.0f} MLUPS, kernel {r['kernel_ms']:.4f} ms")
+    for name in policies:
+        r = headline_leg(xa, torch, xa.PrecisionPolicy[name], n, steps, 5, python_loop=False)
+        log(f"bench: profile pass {name} {r['mlups']:.0f} MLUPS, kernel {r['kernel_ms']:.4f} ms")
 
 
 def _read_pmc(path, counter):
@@ -561,6 +562,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the precision, adjoint and KBC legs")
     ap.add_argument("--quick", action="store_true", help="smaller extra legs")
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--profile-policies", default="FP32FP32",
+                    help="comma-separated precision policies the --profile pass steps (default FP32FP32)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--only", default=None,
                     help="comma-separated extra legs to run alone (kbc_sphere, grads_sphere, les_cavity, mesh_masker, "
@@ -578,7 +581,7 @@ def main():
 
     n = args.size
     if args.profile:
-        profile_pass(xa, torch, n, args.steps)
+        profile_pass(xa, torch, n, args.steps, args.profile_policies.split(","))
         return
     if args.only:
         legs = {
