@@ -204,6 +204,15 @@ def make_bcs(name, grid, xa):
         ]
     if name == "do_nothing":
         return [BC.DoNothingBC(indices=ne["right"]), BC.HalfwayBounceBackBC(indices=ne["left"])]
+    if name == "c2_sphere":  # C2: sphere r = 12 at (64, 64, 64), the rest as the wind tunnel above
+        walls = unique_faces(box, ["bottom", "top", "front", "back"], 3)
+        sph = sphere_indices(shape, (64, 64, 64), 12)
+        return [
+            BC.FullwayBounceBackBC(indices=walls),
+            BC.RegularizedBC("velocity", prescribed_value=(0.04, 0.0, 0.0), indices=ne["left"]),
+            BC.ExtrapolationOutflowBC(indices=ne["right"]),
+            BC.HalfwayBounceBackBC(indices=sph),
+        ]
     if name == "channel2d_zouhe_interior":  # a Zou-He inlet on an interior column (padded: missing[0] set)
         walls = unique_faces(box, ["bottom", "top"], 2)
         col = [[2] * (shape[1] - 2), list(range(1, shape[1] - 1))]
@@ -256,8 +265,12 @@ CASES = {
 LARGE_CASES = {
     # C1: 64^3 lid-driven cavity, halfway walls, lid u = (0.05, 0, 0), Re = 100 (nu = 0.05*64/100)
     "c1_cavity64": ("D3Q19", (64, 64, 64), "BGK", 1.0 / (3 * 0.05 * 64 / 100 + 0.5), "rest"),
+    # C2: D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, inlet 0.04, Re = 200 on the diameter
+    "c2_sphere": ("D3Q27", (256, 128, 128), "KBC", 1.0 / (3 * 0.04 * 24 / 200 + 0.5), "rest"),
+    # C3: periodic D3Q19 BGK Taylor-Green vortex 512x256x256, u0 = 0.04, nu = 0.01
+    "c3_tgv": ("D3Q19", (512, 256, 256), "BGK", 1.0 / (3 * 0.01 + 0.5), "tgv"),
 }
-BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "cavity_smagorinsky": "cavity_halfway",
+BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "c3_tgv": "periodic", "cavity_smagorinsky": "cavity_halfway",
           "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d",
           "sphere3d_grads_bgk": "sphere3d_grads"}
 
@@ -310,7 +323,17 @@ class Pair:
         # oracle state
         self.o_mask, self.o_missing = oracle_masks(self.lat, vs.q, vs.d, shape3, self.bcs)
         f_init = np.zeros((vs.q,) + shape3, dtype=STORE_NP[self.fmt])
-        if init == "random":
+        if init == "tgv":
+            g = [2 * np.pi / s * np.arange(s) for s in shape3]
+            X, Y, Z = np.meshgrid(*g, indexing="ij", sparse=True)
+            u = np.zeros((3,) + shape3)
+            u[0] = 0.04 * np.sin(X) * np.cos(Y) * np.cos(Z)
+            u[1] = -0.04 * np.cos(X) * np.sin(Y) * np.cos(Z)
+            rho = np.ones(shape3)
+            orc.equilibrium(self.lat, self.fmt, self.comp, rho, u, f_init)
+            self.f0.copy_(from_np(f_init, self.f0))
+            self.f1.copy_(self.f0)
+        elif init == "random":
             rho, u = random_init(vs, shape3, seed)
             orc.equilibrium(self.lat, self.fmt, self.comp, rho, u, f_init)
             self.f0.copy_(from_np(f_init, self.f0))

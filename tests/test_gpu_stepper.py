@@ -50,7 +50,7 @@ def stored_ulp(raw, fmt):
     return 2.0 ** (np.floor(np.log2(m)) - mant) if m > 0 else 0.0
 
 
-def compare(pair, steps, tol=None, use_run=False):
+def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None):
     pair.step_product(steps, use_run=use_run)
     pair.step_oracle(steps)
     fa, fb = pair.product_f(), pair.oracle_f()
@@ -64,7 +64,10 @@ def compare(pair, steps, tol=None, use_run=False):
     du = np.abs(ua - ub)[:, ~solid].max()
     if pair.fmt in SIXTEEN:
         ulp = stored_ulp(pair.o_f0, pair.fmt)
-        assert df <= 4 * ulp, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > 4 ulps ({4 * ulp:.3e})"
+        assert df <= ulps * ulp, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > {ulps} ulps ({ulps * ulp:.3e})"
+        if mean_ulps is not None:
+            mean = np.abs(fa - fb).mean()
+            assert mean <= mean_ulps * ulp, f"{pair.name} {pair.policy}: mean|df| = {mean:.3e} > {mean_ulps} ulps"
         # rho and u sum Q stored values, each of which may be a few ulps off
         tm = max(tm, 4 * ulp * np.sqrt(pair.vs.q))
     elif pair.policy == P.FP32FP32 and (df > tf or dr > tm or du > tm):
@@ -98,6 +101,27 @@ def test_c1_cavity64_1000_steps_matches_oracle(policy):
     print(f"C1 {policy.name}: max|drho| {dr:.2e} max|du| {du:.2e} max|df| {df:.2e} (max|u| {np.abs(ub).max():.3f})")
     bound = 1e-5 if policy == P.FP32FP32 else 1e-12
     assert dr <= bound and du <= bound
+
+
+def test_c2_full_size_matches_oracle():
+    """C2 at its full size (D3Q27 KBC wind tunnel 256x128x128, sphere r = 12): 20 FP32 steps
+    against the oracle, with the stepper tolerances above."""
+    pair = Pair("c2_sphere", P.FP32FP32)
+    df, dr, du = compare(pair, 20)
+    print(f"C2 full size, 20 steps: max|df| {df:.2e} max|drho| {dr:.2e} max|du| {du:.2e}")
+
+
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP32FP16_SHIFTED])
+def test_c3_full_size_matches_oracle(policy):
+    """C3 at its full size (periodic Taylor-Green vortex 512x256x256): 10 steps against the
+    oracle in FP32 and in shifted FP16 storage.  For FP16 the two sides round independently (FMA
+    contraction on the GPU only) and a one-ulp difference is carried and mixed by later steps; over
+    6.4e8 stored values the worst one reaches a few ulps (measured 4.25), so the elementwise bound
+    is 8 ulps of the field's largest stored magnitude, with the mean difference below a quarter of
+    that ulp (measured 0.14: most values are far smaller than the largest)."""
+    pair = Pair("c3_tgv", policy)
+    df, dr, du = compare(pair, 10, ulps=8, mean_ulps=0.25)
+    print(f"C3 full size {policy.name}, 10 steps: max|df| {df:.2e} max|drho| {dr:.2e} max|du| {du:.2e}")
 
 
 @pytest.mark.parametrize("name", ["periodic", "cavity_fullway", "channel2d_zouhe", "channel2d_regularized", "sphere3d",
