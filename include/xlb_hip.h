@@ -242,6 +242,11 @@ int64_t xlbh_mesh_workspace_bytes(int nx, int ny, int nz);
 int xlbh_mesh_mask(int lattice, const float* triangles, int64_t n_triangles, int bc_id, uint8_t* bc_mask,
                    uint8_t* missing_bool, void* workspace, int nx, int ny, int nz, void* stream);
 
+/* Elementwise cast of n values from field type `field_in` to `field_out` (XLBH_FIELD_*), the
+ * reference's PrecisionCaster (xlb/operator/precision_caster/precision_caster.py:43-82).  Each
+ * value is converted with one rounding, except FP64 -> FP16 / BF16, which rounds to FP32 first. */
+int xlbh_cast(int field_in, int field_out, const void* in, void* out, int64_t n, void* stream);
+
 /* Post-processing of a velocity field u (3, nx, ny, nz) of type `field` (F32 or F64): over the
  * interior cells, skipping every cell with a non-zero boundary id among its six face neighbours
  * (bc_mask may be NULL), central differences give

@@ -23,6 +23,7 @@ import xlb_amd.operator.boundary_condition
 import xlb_amd.operator.boundary_masker
 import xlb_amd.operator.force
 import xlb_amd.operator.postprocess
+import xlb_amd.operator.precision_caster
 import xlb_amd.operator.stepper
 import xlb_amd.helper
 import xlb_amd.autodiff

@@ -71,6 +71,7 @@ SIGNATURES = {
     "xlbh_compact_cells": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_scatter_aux": (_i, [_i, _vp, _vp, _i, _vp, _vp]),
     "xlbh_mesh_workspace_bytes": (_i64, [_i, _i, _i]),
+    "xlbh_cast": (_i, [_i, _i, _vp, _vp, _i64, _vp]),
     "xlbh_vorticity": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_q_criterion": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_mesh_mask": (_i, [_i, _vp, _i64, _i, _vp, _vp, _vp, _i, _i, _i, _vp]),

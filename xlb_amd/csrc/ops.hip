@@ -294,6 +294,23 @@ int with_field(int field, Fn&& fn) {
   return 1;
 }
 
+// ------------------------------------------------------------------------------------------
+// Elementwise precision cast (precision_caster.py:43-82).  Values go through float unless both
+// ends are double; 16-bit ends use the same conversions as the population formats.
+// ------------------------------------------------------------------------------------------
+template <class I, class O>
+__global__ __launch_bounds__(256) void k_cast(const I* __restrict__ in, O* __restrict__ out, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    if constexpr (__is_same(I, double) && __is_same(O, double))
+      out[i] = in[i];
+    else if constexpr (__is_same(O, double))
+      out[i] = (double)raw_to<I, float>(in[i]);
+    else
+      out[i] = to_raw<O, float>(raw_to<I, float>(in[i]));
+  }
+}
+
 }  // namespace xlbh
 
 using namespace xlbh;
@@ -457,6 +474,24 @@ int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id, cons
       using R = typename Fmt<FMT>::raw;
       hipLaunchKernelGGL((k_apply_bc<LAT, FMT, C>), sweep_grid(sw), dim3(256), 0, s, kind, bc_id, d,
                          (const R*)f_pre, (R*)f_post, bc_mask, missing_bits, (const C*)aux, sw, N);
+      XLBH_LAUNCH_CHECK();
+      return 0;
+    });
+  });
+}
+
+int xlbh_cast(int field_in, int field_out, const void* in, void* out, int64_t n, void* stream) {
+  clear_error();
+  XLBH_REQUIRE(n >= 0, "negative element count");
+  XLBH_REQUIRE(n == 0 || (in && out), "cast needs input and output");
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20);
+  return with_field(field_in, [&](auto a) -> int {
+    using I = decltype(a);
+    return with_field(field_out, [&](auto b) -> int {
+      using O = decltype(b);
+      hipLaunchKernelGGL((k_cast<I, O>), dim3(blocks), dim3(256), 0, s, (const I*)in, (O*)out, n);
       XLBH_LAUNCH_CHECK();
       return 0;
     });
