@@ -9,7 +9,9 @@ the mathematics instead:
 * FP32 adjoints against FP64 adjoints of the same trajectory;
 * checkpoint spacing (every step, every k, sqrt(N), none) changes nothing but summation order;
   BF16 checkpoints of f - w stay within a stated bound.
-Relative error bar: 1e-6 for FP64 against FD (the target for the method is 1e-4).
+Relative error bars for FP64 against central FD (eps = 1e-6): 1e-8 of the largest entry for
+elementwise checks and 2e-8 for directional ones; the measured worst cases are 1.2e-9 and 5.5e-9,
+the FD round-off level (the target for the method is 1e-4).
 """
 
 import numpy as np
@@ -23,6 +25,8 @@ from lbm_cases import make_bcs
 pytestmark = pytest.mark.gpu
 
 P = xa.PrecisionPolicy
+FD_ELEMENT_TOL = 1e-8
+FD_DIRECTION_TOL = 2e-8
 
 ADJ_CASES = {
     # name: (lattice, shape, boundary set, omega, force)
@@ -127,7 +131,8 @@ def test_adjoint_elementwise_fd(name):
         fd.view(-1)[i] = (c.loss_at(fp, steps) - c.loss_at(fm, steps)) / (2 * eps)
     scale = fd.abs().max().item()
     err = (g_f - fd).abs().max().item()
-    assert err <= 1e-6 * scale, f"{name}: max|adj - fd| = {err:.3e} (scale {scale:.3e})"
+    print(f"{name}: f0 max|adj - fd| / scale = {err / scale:.2e}")
+    assert err <= FD_ELEMENT_TOL * scale, f"{name}: max|adj - fd| = {err:.3e} (scale {scale:.3e})"
     if c.ff is not None:
         fdg = torch.zeros_like(g_g)
         for i in range(c.ff.numel()):
@@ -138,7 +143,8 @@ def test_adjoint_elementwise_fd(name):
         c.st.collision.force_field = c.ff
         scale = fdg.abs().max().item()
         err = (g_g - fdg).abs().max().item()
-        assert err <= 1e-6 * scale, f"{name}: force grad max|adj - fd| = {err:.3e} (scale {scale:.3e})"
+        print(f"{name}: force max|adj - fd| / scale = {err / scale:.2e}")
+        assert err <= FD_ELEMENT_TOL * scale, f"{name}: force grad max|adj - fd| = {err:.3e} (scale {scale:.3e})"
 
 
 def _near_boundary(c):
@@ -175,13 +181,15 @@ def test_adjoint_directional_fd(name):
     for tag, v in _directions(c.f0.shape, near, 3):
         fd = (c.loss_at(c.f0 + eps * v, steps) - c.loss_at(c.f0 - eps * v, steps)) / (2 * eps)
         ad = float((g_f * v).sum())
-        assert rel(ad, fd) <= 1e-6, f"{name} f0/{tag}: adjoint {ad:.10e} vs FD {fd:.10e}"
+        print(f"{name} f0/{tag}: rel {rel(ad, fd):.2e}")
+        assert rel(ad, fd) <= FD_DIRECTION_TOL, f"{name} f0/{tag}: adjoint {ad:.10e} vs FD {fd:.10e}"
     if c.ff is not None:
         for tag, v in _directions(c.ff.shape, near, 5):
             fd = (c.loss_at(c.f0, steps, c.ff + eps * v) - c.loss_at(c.f0, steps, c.ff - eps * v)) / (2 * eps)
             c.st.collision.force_field = c.ff
             ad = float((g_g * v).sum())
-            assert rel(ad, fd) <= 1e-6, f"{name} force/{tag}: adjoint {ad:.10e} vs FD {fd:.10e}"
+            print(f"{name} force/{tag}: rel {rel(ad, fd):.2e}")
+            assert rel(ad, fd) <= FD_DIRECTION_TOL, f"{name} force/{tag}: adjoint {ad:.10e} vs FD {fd:.10e}"
 
 
 @pytest.mark.parametrize("name", ["channel2d_zouhe", "sphere3d", "periodic3d_uniform"])
