@@ -14,7 +14,8 @@ Also reported on the same line:
   steps: relative L2 velocity error and kinetic-energy decay of every storage format against an
   FP64/FP64 run of the same kernel (itself matched to the FP64 CPU oracle to 1e-12 by the tests);
 * adjoint -- backward (checkpoint recompute + adjoint kernels) over forward time per step for the
-  differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50) and the 256^3 cavity;
+  differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50), the 256^3 cavity and the
+  128^3 decaying turbulence (2000 steps, BF16 checkpoints every 45);
 * kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, with the momentum-exchange drag
   every 100 steps inside the timed loop; grads_sphere -- the same with a GradsApproximationBC
   sphere;
@@ -266,7 +267,64 @@ def adjoint_leg(xa, torch, quick=False):
     log("bench: adjoint cavity3d: " + ", ".join(f"{k}={v:.4g}" for k, v in r.items() if k != "workload"))
     del wl
     torch.cuda.empty_cache()
+
+    res["turbulence3d"] = turbulence_adjoint(xa, torch, quick)
     return res
+
+
+def turbulence_adjoint(xa, torch, quick=False):
+    """C5: gradient of the band-limited kinetic energy after 2000 steps of 128^3 decaying turbulence
+    with respect to the initial velocity (through the equilibrium initialisation), FP32 with BF16
+    checkpoints every sqrt(N) steps.  Backward / forward time per step, best of three."""
+    from xlb_amd.autodiff import equilibrium_torch, macroscopic_torch, rollout
+    from xlb_amd.workloads import band_energy, decaying_turbulence_3d, turbulence_velocity
+
+    n, steps = (64, 400) if quick else (128, 2000)
+    wl = decaying_turbulence_3d(n, xa.PrecisionPolicy.FP32FP32, omega=1.95)
+    st, vs = wl.stepper, wl.stepper.velocity_set
+    u_init = turbulence_velocity(n, seed=3)
+    rho = torch.ones((1, n, n, n), dtype=torch.float64, device="cuda")
+
+    def start(u):
+        return equilibrium_torch(rho, u, vs).to(st.store_dtype)
+
+    def loss(fT):
+        _, u = macroscopic_torch(fT.double(), vs)
+        return band_energy(u)
+
+    every = int(np.ceil(np.sqrt(steps)))
+    f0 = start(u_init).contiguous()
+    f1 = torch.empty_like(f0)
+    st.run(f0.clone(), f1, wl.bc_mask, wl.missing_mask, wl.omega, steps)  # warm-up
+    t_run = float("inf")
+    for _ in range(3):
+        a = f0.clone()
+        t, _ = _sync_time(torch, lambda: st.run(a, f1, wl.bc_mask, wl.missing_mask, wl.omega, steps))
+        t_run = min(t_run, t)
+    u0 = u_init.clone().requires_grad_(True)
+    loss(rollout(st, start(u0), wl.bc_mask, wl.missing_mask, wl.omega, 2 * every, checkpoint_every=every,
+                 checkpoint_dtype="bf16")).backward()  # untimed warm-up of the reverse pass
+    t_roll = t_bwd = float("inf")
+    for _ in range(3):
+        u0 = u_init.clone().requires_grad_(True)
+        t, fT = _sync_time(torch, lambda: rollout(st, start(u0), wl.bc_mask, wl.missing_mask, wl.omega, steps,
+                                                   checkpoint_every=every, checkpoint_dtype="bf16"))
+        t_roll = min(t_roll, t)
+        L = loss(fT)
+        torch.cuda.synchronize()
+        t, _ = _sync_time(torch, lambda: L.backward())
+        t_bwd = min(t_bwd, t)
+        del fT, L
+    r = dict(fwd_ms_per_step=t_run / steps * 1e3, rollout_fwd_ms_per_step=t_roll / steps * 1e3,
+             bwd_ms_per_step=t_bwd / steps * 1e3, ratio_bwd_over_fwd=t_bwd / t_run,
+             ratio_fwd_plus_bwd_over_fwd=(t_roll + t_bwd) / t_run, grad_norm=float(u0.grad.norm()),
+             workload=(f"D3Q19 BGK decaying turbulence {n}^3 periodic, k^-5/3 band 2..8, rms 0.03, {steps} steps, "
+                       f"BF16 checkpoints every {every}, FP32; gradient of the 4..8 band energy w.r.t. the initial "
+                       "velocity"))
+    log("bench: adjoint turbulence3d: " + ", ".join(f"{k}={v:.4g}" for k, v in r.items() if k != "workload"))
+    del wl, f0, f1
+    torch.cuda.empty_cache()
+    return r
 
 
 def kbc_sphere_leg(xa, torch, steps=500, quick=False, sphere_bc="halfway"):
