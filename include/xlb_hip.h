@@ -261,7 +261,8 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
                      int nx, int ny, int nz, void* stream);
 
 /* ---------------------------------------------------------------------------------------
- * Reverse mode of the fused step (BGK, all boundary kinds).  No counterpart function in the
+ * Reverse mode of the fused step (BGK and Smagorinsky BGK, every boundary kind; `collision` and
+ * `collision_param` as for xlbh_step, KBC is rejected).  No counterpart function in the
  * reference, whose gradients come from Warp's tape replaying generated adjoint kernels
  * (examples/out_of_core/subroutine/autodiff_stepper_subroutine.py:459-466).
  *
@@ -276,15 +277,16 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
  * Other arguments as for xlbh_step.  Solid cells (id 255), which the forward step leaves
  * unwritten, pass nothing back.
  * ------------------------------------------------------------------------------------- */
-int xlbh_adjoint_step(int lattice, int store, int compute,
+int xlbh_adjoint_step(int lattice, int collision, int store, int compute,
                       const void* f_in, const void* lam_out, void* lam_in,
                       const uint8_t* bc_mask, const uint32_t* missing_bits,
                       const void* aux, const xlbh_bc_desc* bc_table,
                       const int32_t* bc_cells, int n_bc_cells,
                       int force_mode, const double* force_uniform, const void* force_field,
-                      double* grad_force, int nx, int ny, int nz, double omega, void* stream);
+                      double* grad_force, int nx, int ny, int nz, double omega, double collision_param,
+                      void* stream);
 
-/* Reverse sweep over one checkpoint segment of n_steps BGK steps, below the ABI (the counterpart
+/* Reverse sweep over one checkpoint segment of n_steps steps, below the ABI (the counterpart
  * of xlbh_run for the backward pass; the reference replays a Warp tape per segment,
  * autodiff_stepper_subroutine.py:459-466).
  *   f_ckpt : the state f^{t0} at the segment start (store format), read only.
@@ -302,14 +304,14 @@ int xlbh_adjoint_step(int lattice, int store, int compute,
  *   the same workspace to every call, with XLBH_ADJ_BEGIN on its first call (clears it) and
  *   XLBH_ADJ_END on its last (adds the carried contributions into the result).  Calls that share
  *   one workspace must not run concurrently. */
-int xlbh_adjoint_run(int lattice, int store, int compute,
+int xlbh_adjoint_run(int lattice, int collision, int store, int compute,
                      const void* f_ckpt, void* states, void* lam_a, void* lam_b,
                      const uint8_t* bc_mask, const uint32_t* missing_bits,
                      const void* aux, const xlbh_bc_desc* bc_table,
                      const int32_t* bc_cells, int n_bc_cells,
                      int force_mode, const double* force_uniform, const void* force_field,
-                     double* grad_force, int nx, int ny, int nz, double omega, int timestep0,
-                     int n_steps, void* workspace, int flags, void* stream);
+                     double* grad_force, int nx, int ny, int nz, double omega, double collision_param,
+                     int timestep0, int n_steps, void* workspace, int flags, void* stream);
 
 #define XLBH_ADJ_RECOMPUTE 1
 #define XLBH_ADJ_BEGIN 2

@@ -39,6 +39,8 @@ ADJ_CASES = {
     "tiny3d_grads_sphere": ("D3Q19", (8, 6, 6), "sphere3d_grads", 1.6, "field"),
     "tiny3d_grads_cavity": ("D3Q19", (5, 5, 4), "cavity_grads", 1.4, None),
     "tiny2d_zouhe_interior": ("D2Q9", (8, 6), "channel2d_zouhe_interior", 1.2, "field"),
+    "tiny3d_smagorinsky": ("D3Q19", (6, 5, 5), "cavity_halfway", 1.9, "field", "SmagorinskyLESBGK"),
+    "tiny2d_smagorinsky": ("D2Q9", (8, 6), "channel2d_regularized", 1.95, "uniform", "SmagorinskyLESBGK"),
     "periodic2d": ("D2Q9", (24, 20), "periodic", 1.3, "field"),
     "periodic3d_uniform": ("D3Q19", (10, 8, 6), "periodic", 1.6, "uniform"),
     "channel2d_zouhe": ("D2Q9", (32, 16), "channel2d_zouhe", 1.2, "field"),
@@ -50,6 +52,8 @@ ADJ_CASES = {
     "do_nothing": ("D3Q19", (12, 10, 10), "do_nothing", 1.0, "field"),
     "grads_sphere": ("D3Q19", (24, 16, 16), "sphere3d_grads", 1.6, "field"),
     "grads_sphere_q27": ("D3Q27", (20, 12, 12), "sphere3d_grads", 1.6, "field"),
+    "smagorinsky_sphere": ("D3Q19", (24, 16, 16), "sphere3d", 1.9, "field", "SmagorinskyLESBGK"),
+    "smagorinsky_q27": ("D3Q27", (16, 12, 12), "sphere3d_grads", 1.95, None, "SmagorinskyLESBGK"),
 }
 
 TINY = [k for k in ADJ_CASES if k.startswith("tiny")]
@@ -58,7 +62,8 @@ LARGER = [k for k in ADJ_CASES if not k.startswith("tiny")]
 
 class Case:
     def __init__(self, name, policy, seed=0):
-        lat, shape, bcname, omega, force = ADJ_CASES[name]
+        lat, shape, bcname, omega, force = ADJ_CASES[name][:5]
+        collision = ADJ_CASES[name][5] if len(ADJ_CASES[name]) > 5 else "BGK"
         vs = getattr(xa.velocity_set, lat)(policy, xa.ComputeBackend.MI355X)
         xa.init(velocity_set=vs, default_backend=xa.ComputeBackend.MI355X, default_precision_policy=policy)
         grid = xa.grid.grid_factory(shape, compute_backend=xa.ComputeBackend.MI355X)
@@ -73,7 +78,7 @@ class Case:
         elif force == "uniform":
             fv = np.array([1e-4, -2e-4, 5e-5][: vs.d])
         self.st = xa.operator.stepper.IncompressibleNavierStokesStepper(
-            grid, bcs, collision_type="BGK", force_vector=fv, force_field=self.ff
+            grid, bcs, collision_type=collision, force_vector=fv, force_field=self.ff
         )
         _, _, self.bc_mask, self.missing = self.st.prepare_fields()
         # generic state: equilibrium of a random flow plus a non-equilibrium part
@@ -232,6 +237,14 @@ def test_adjoint_rejects_bad_arguments():
     kbc = xa.operator.stepper.IncompressibleNavierStokesStepper(c.st.grid, [], collision_type="KBC")
     with pytest.raises(NotImplementedError):
         kbc.adjoint_step(c.f0, lam, torch.zeros_like(lam), c.bc_mask, c.missing, c.omega)
+    # and the C ABI rejects KBC itself
+    from xlb_amd import _lib
+    nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = c.st._launch_args(c.f0, None, c.bc_mask, c.missing)
+    rc = _lib.lib().xlbh_adjoint_step(c.st.lattice_id, _lib.KBC, c.st.precision_policy.store_format, c.st.compute_id,
+                                      _lib.ptr(c.f0), _lib.ptr(lam), _lib.ptr(torch.zeros_like(lam)), *masks, None, 0,
+                                      force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz, float(c.omega), 0.0,
+                                      _lib.stream_of(c.f0))
+    assert rc != 0 and b"Smagorinsky" in _lib.lib().xlbh_last_error()
 
 
 @pytest.mark.parametrize("name", ["tiny2d_channel", "tiny2d_regularized", "sphere3d", "do_nothing", "grads_sphere"])
@@ -250,9 +263,9 @@ def test_boundary_pass_list_equals_grid_scan(name):
     via_scan = torch.empty_like(lam_out)
     nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
     _lib.check(_lib.lib().xlbh_adjoint_step(
-        st.lattice_id, st.precision_policy.store_format, st.compute_id, _lib.ptr(c.f0), _lib.ptr(lam_out),
-        _lib.ptr(via_scan), *masks, None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz, float(c.omega),
-        _lib.stream_of(c.f0)))
+        st.lattice_id, st._collision_id(), st.precision_policy.store_format, st.compute_id, _lib.ptr(c.f0),
+        _lib.ptr(lam_out), _lib.ptr(via_scan), *masks, None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
+        float(c.omega), st._collision_param(), _lib.stream_of(c.f0)))
     torch.cuda.synchronize()
     assert st._bc_cells is not None and st._bc_cells.numel() > 0
     err = (via_list - via_scan).abs().max().item()
@@ -261,7 +274,8 @@ def test_boundary_pass_list_equals_grid_scan(name):
 
 @pytest.mark.parametrize("name", ["channel2d_zouhe", "channel2d_regularized", "tiny2d_zouhe_pressure", "sphere3d",
                                   "sphere3d_q27", "do_nothing", "cavity_fullway", "grads_sphere", "grads_sphere_q27",
-                                  "tiny3d_grads_cavity", "tiny2d_zouhe_interior"])
+                                  "tiny3d_grads_cavity", "tiny2d_zouhe_interior", "smagorinsky_sphere",
+                                  "smagorinsky_q27"])
 def test_fused_and_split_reverse_agree(name):
     """The fused one-kernel reverse step (boundary contributions carried in the workspace, Zou-He /
     regularized Jacobian by the hand-written one-pass adjoint) against the two-kernel reverse step

@@ -33,7 +33,7 @@ BC_GRADS_APPROXIMATION = 10
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 ADJ_RECOMPUTE, ADJ_BEGIN, ADJ_END = 1, 2, 4
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class BcDesc(ctypes.Structure):
@@ -78,10 +78,10 @@ SIGNATURES = {
     "xlbh_reduce_workspace_bytes": (_i64, [_i64]),
     "xlbh_momentum_transfer": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_mass_momentum": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
-    "xlbh_adjoint_step": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i, _i,
-                               _d, _vp]),
-    "xlbh_adjoint_run": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
-                              _i, _d, _i, _i, _vp, _i, _vp]),
+    "xlbh_adjoint_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
+                               _i, _d, _d, _vp]),
+    "xlbh_adjoint_run": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i,
+                              _i, _i, _d, _d, _i, _i, _vp, _i, _vp]),
     "xlbh_adjoint_workspace_bytes": (_i64, [_i, _i, _i, _i, _i]),
     "xlbh_record": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _d, _d, _i, _i, _vp]),
 }

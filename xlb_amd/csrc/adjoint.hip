@@ -1,4 +1,5 @@
-// Reverse mode of the fused step (BGK): given the state f^t that a forward step read and
+// Reverse mode of the fused step (BGK, and Smagorinsky BGK through collide_adjoint): given the
+// state f^t that a forward step read and
 // lam_out = dL/df^{t+1}, produce lam_in = dL/df^t and accumulate dL/dg for a per-cell body force.
 //
 // Forward per cell x (step.hpp, nse_stepper.py:344-383):  f_ps(x) = pull(f^t)  ->  h = BC_stream(f_ps)
@@ -80,11 +81,52 @@ __device__ __forceinline__ void eq_moments(const C* lam, const C* u, const C* v,
   (void)u;
 }
 
+// Smagorinsky BGK (physics.hpp, smagorinsky_les_bgk.py:73-89) relaxes with k = 1/tau(S) instead of
+// omega, S = sum_l a_l fneq_l^2 (a_l = 1 where the signed component sum of c_l is 1, 2 where it is
+// >= 2): the BGK adjoint with k for omega plus the strain term
+//   -(sum_l G_l fneq_l) dk/dS dS/dh_m,  dS/dh_m = 2 a_m fneq_m - (A + B.(c_m - u))[2 a fneq],
+//   dk/dS = -k^2 9 Cs^2 / (2 sqrt(tau0^2 + 36 Cs^2 sqrt S) sqrt S)  (taken as 0 at S = 0).
 template <int LAT, class C>
-__device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, bool has_force, const C* F, C* gg) {
+__device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, bool has_force, const C* F, C* gg,
+                                                bool smag = false, C smag_cs = C(0)) {
   constexpr LatTables T = make_tables(LAT);
   C rho, u[3];
   macroscopic<LAT>(h, rho, u);
+  C strain[T.q];
+#pragma unroll
+  for (int k = 0; k < T.q; ++k) strain[k] = C(0);
+  if (smag) {
+    const C usqr = usqr15<LAT>(u);
+    C fneq[T.q], lamS[T.q];
+    C S = C(0), Gf = C(0);
+#pragma unroll
+    for (int l = 0; l < T.q; ++l) {
+      const int csum = T.c[0][l] + T.c[1][l] + T.c[2][l];
+      const C al = csum == 1 ? C(1) : (csum >= 2 ? C(2) : C(0));
+      fneq[l] = h[l] - feq_l<LAT>(l, rho, u, usqr);
+      S += al * fneq[l] * fneq[l];
+      lamS[l] = C(2) * al * fneq[l];
+      Gf += lam[l] * fneq[l];
+    }
+    const C tau0 = C(1) / omega;
+    const C sqS = sqrt(S);
+    const C root = sqrt(tau0 * tau0 + C(36) * (smag_cs * smag_cs) * sqS);
+    const C tau = tau0 + C(0.5) * (root - tau0);
+    const C kk = C(1) / tau;
+    if (S > C(0)) {
+      const C dkdS = -kk * kk * C(9) * (smag_cs * smag_cs) / (C(2) * root * sqS);
+      C AS, BS[3];
+      eq_moments<LAT>(lamS, u, u, AS, BS);
+#pragma unroll
+      for (int m = 0; m < T.q; ++m) {
+        C dS = lamS[m] - AS;
+#pragma unroll
+        for (int d = 0; d < T.d; ++d) dS -= BS[d] * (C(T.c[d][m]) - u[d]);
+        strain[m] = -Gf * dkdS * dS;
+      }
+    }
+    omega = kk;
+  }
   const C a = has_force ? omega - C(1) : omega;
   C A0, B0[3];
   eq_moments<LAT>(lam, u, u, A0, B0);
@@ -102,7 +144,7 @@ __device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, boo
     C s = a * A0 + A1;
 #pragma unroll
     for (int d = 0; d < T.d; ++d) s += (a * B0[d] + B1[d]) * (C(T.c[d][k]) - u[d]);
-    lam[k] = (C(1) - omega) * lam[k] + s;
+    lam[k] = (C(1) - omega) * lam[k] + s + strain[k];
   }
 }
 
@@ -274,6 +316,8 @@ struct AdjParams {
   double fu[3];
   int force_mode;
   double omega;
+  double cparam;   // Smagorinsky coefficient
+  int collision;   // XLBH_BGK or XLBH_SMAGORINSKY_BGK
   Sweep sw;
   uint32_t N;
 };
@@ -391,7 +435,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
     if (kind == XLBH_BC_EQUILIBRIUM && p.force_mode != XLBH_FORCE_NONE && p.grad_force) {
       C F[3], gg[3];
       cell_force<LAT>(p, cell, F);
-      collide_adjoint<LAT>(h, lam, (C)p.omega, true, F, gg);
+      collide_adjoint<LAT>(h, lam, (C)p.omega, true, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
     }
@@ -401,7 +445,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
     const bool has_force = p.force_mode != XLBH_FORCE_NONE;
 #pragma unroll
     for (int k = 0; k < Q; ++k) G[k] = lam[k];
-    collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg);
+    collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
     if (has_force && p.grad_force) {
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
@@ -448,7 +492,7 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
   C G[Q];
 #pragma unroll
   for (int k = 0; k < Q; ++k) G[k] = lam[k];
-  collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg);
+  collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
 
   if (part <= 0) {
     if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
@@ -619,7 +663,7 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
     const bool has_force = p.force_mode != XLBH_FORCE_NONE;
 #pragma unroll
     for (int k = 0; k < Q; ++k) Gh[k] = lam[k];
-    collide_adjoint<LAT>(h, Gh, (C)p.omega, has_force, F, gg);
+    collide_adjoint<LAT>(h, Gh, (C)p.omega, has_force, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
     if (has_force && p.grad_force) {
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
@@ -822,10 +866,12 @@ static int adjoint_dispatch(int lattice, int store, int compute, const AdjParams
   return 1;
 }
 
-static int adjoint_validate(int lattice, int force_mode, const double* force_uniform, const void* force_field,
-                            const uint8_t* bc_mask, const uint32_t* missing_bits, const xlbh_bc_desc* bc_table,
-                            int nx, int ny, int nz) {
+static int adjoint_validate(int lattice, int collision, int force_mode, const double* force_uniform,
+                            const void* force_field, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                            const xlbh_bc_desc* bc_table, int nx, int ny, int nz) {
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_SMAGORINSKY_BGK,
+               "the adjoint supports BGK and Smagorinsky BGK collisions");
   XLBH_REQUIRE(!bc_mask || (missing_bits && bc_table), "bc_mask given without missing bits or BC table");
   XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
   XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
@@ -834,11 +880,14 @@ static int adjoint_validate(int lattice, int force_mode, const double* force_uni
   return 0;
 }
 
-static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam_in, const uint8_t* bc_mask,
-                                const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
-                                const int32_t* bc_cells, int n_bc_cells, int force_mode, const double* force_uniform, const void* force_field,
-                                double* grad_force, int nx, int ny, int nz, double omega) {
+static AdjParams adjoint_params(int collision, double cparam, const void* f_in, const void* lam_out, void* lam_in,
+                                const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                                const xlbh_bc_desc* bc_table, const int32_t* bc_cells, int n_bc_cells, int force_mode,
+                                const double* force_uniform, const void* force_field, double* grad_force, int nx,
+                                int ny, int nz, double omega) {
   AdjParams p;
+  p.collision = collision;
+  p.cparam = cparam;
   p.f_in = f_in;
   p.lam_out = lam_out;
   p.lam_in = lam_in;
@@ -862,15 +911,16 @@ static AdjParams adjoint_params(const void* f_in, const void* lam_out, void* lam
   return p;
 }
 
-extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void* f_ckpt, void* states, void* lam_a,
-                                void* lam_b, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
-                                const xlbh_bc_desc* bc_table, const int32_t* bc_cells, int n_bc_cells,
-                                int force_mode, const double* force_uniform,
+extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compute, const void* f_ckpt, void* states,
+                                void* lam_a, void* lam_b, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                                const void* aux, const xlbh_bc_desc* bc_table, const int32_t* bc_cells,
+                                int n_bc_cells, int force_mode, const double* force_uniform,
                                 const void* force_field, double* grad_force, int nx, int ny, int nz, double omega,
-                                int timestep0, int n_steps, void* workspace, int flags, void* stream) {
+                                double collision_param, int timestep0, int n_steps, void* workspace, int flags,
+                                void* stream) {
   clear_error();
-  if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
-                               ny, nz))
+  if (int e = adjoint_validate(lattice, collision, force_mode, force_uniform, force_field, bc_mask, missing_bits,
+                               bc_table, nx, ny, nz))
     return e;
   XLBH_REQUIRE(n_steps >= 1, "n_steps must be >= 1");
   XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
@@ -885,9 +935,9 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
   // recompute f^{t0+1} .. f^{t0+n-1} (unless the caller recorded them, e.g. with xlbh_record)
   const bool recompute = (flags & XLBH_ADJ_RECOMPUTE) != 0;
   for (int j = 1; recompute && j < n_steps; ++j) {
-    if (int e = xlbh_step(lattice, XLBH_BGK, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
-                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega, 0.0,
-                          timestep0 + j - 1, stream))
+    if (int e = xlbh_step(lattice, collision, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
+                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega,
+                          collision_param, timestep0 + j - 1, stream))
       return e;
   }
   void* cur = lam_a;
@@ -899,8 +949,8 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
   char* ws = static_cast<char*>(workspace);
   uint8_t* flag = ws ? reinterpret_cast<uint8_t*>(ws + 3 * qn * cb) : nullptr;
   auto fused_params = [&](const void* f, const void* lo, void* li, int t) {
-    AdjParams p = adjoint_params(f, lo, li, bc_mask, missing_bits, aux, bc_table, bc_cells, n_bc_cells, force_mode,
-                                 force_uniform, force_field, grad_force, nx, ny, nz, omega);
+    AdjParams p = adjoint_params(collision, collision_param, f, lo, li, bc_mask, missing_bits, aux, bc_table, bc_cells,
+                                 n_bc_cells, force_mode, force_uniform, force_field, grad_force, nx, ny, nz, omega);
     p.d_own = ws;
     p.d_nb_out = ws + (1 + (t & 1)) * qn * cb;
     p.d_nb_in = ws + (1 + ((t + 1) & 1)) * qn * cb;
@@ -917,9 +967,9 @@ extern "C" int xlbh_adjoint_run(int lattice, int store, int compute, const void*
       const AdjParams p = fused_params(state(j), cur, nxt, timestep0 + j);
       if (int e = adjoint_dispatch(lattice, store, compute, p, s, ADJ_FUSED)) return e;
     } else {
-      const AdjParams p = adjoint_params(state(j), cur, nxt, bc_mask, missing_bits, aux, bc_table, bc_cells,
-                                         n_bc_cells, force_mode, force_uniform, force_field, grad_force, nx, ny, nz,
-                                         omega);
+      const AdjParams p = adjoint_params(collision, collision_param, state(j), cur, nxt, bc_mask, missing_bits, aux,
+                                         bc_table, bc_cells, n_bc_cells, force_mode, force_uniform, force_field,
+                                         grad_force, nx, ny, nz, omega);
       if (int e = adjoint_dispatch(lattice, store, compute, p, s)) return e;
     }
     void* t = cur;
@@ -940,21 +990,21 @@ extern "C" int64_t xlbh_adjoint_workspace_bytes(int lattice, int compute, int nx
   return ((3 * (int64_t)lattice_q(lattice) * n * cb + n + 255) / 256) * 256;
 }
 
-extern "C" int xlbh_adjoint_step(int lattice, int store, int compute, const void* f_in, const void* lam_out,
-                                 void* lam_in, const uint8_t* bc_mask, const uint32_t* missing_bits,
-                                 const void* aux, const xlbh_bc_desc* bc_table, const int32_t* bc_cells,
-                                 int n_bc_cells, int force_mode, const double* force_uniform,
-                                 const void* force_field, double* grad_force, int nx,
-                                 int ny, int nz, double omega, void* stream) {
+extern "C" int xlbh_adjoint_step(int lattice, int collision, int store, int compute, const void* f_in,
+                                 const void* lam_out, void* lam_in, const uint8_t* bc_mask,
+                                 const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
+                                 const int32_t* bc_cells, int n_bc_cells, int force_mode,
+                                 const double* force_uniform, const void* force_field, double* grad_force, int nx,
+                                 int ny, int nz, double omega, double collision_param, void* stream) {
   clear_error();
-  if (int e = adjoint_validate(lattice, force_mode, force_uniform, force_field, bc_mask, missing_bits, bc_table, nx,
-                               ny, nz))
+  if (int e = adjoint_validate(lattice, collision, force_mode, force_uniform, force_field, bc_mask, missing_bits,
+                               bc_table, nx, ny, nz))
     return e;
   XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
   XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
   XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
-  const AdjParams p = adjoint_params(f_in, lam_out, lam_in, bc_mask, missing_bits, aux, bc_table, bc_cells,
-                                     n_bc_cells, force_mode,
-                                     force_uniform, force_field, grad_force, nx, ny, nz, omega);
+  const AdjParams p = adjoint_params(collision, collision_param, f_in, lam_out, lam_in, bc_mask, missing_bits, aux,
+                                     bc_table, bc_cells, n_bc_cells, force_mode, force_uniform, force_field,
+                                     grad_force, nx, ny, nz, omega);
   return adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream);
 }

@@ -56,7 +56,7 @@ extern "C" {
 
 const char* xlbh_last_error(void) { return g_last_error.c_str(); }
 
-int xlbh_abi_version(void) { return 2; }
+int xlbh_abi_version(void) { return 3; }
 
 int xlbh_step(int lattice, int collision, int store, int compute, const void* f_in, void* f_out,
               const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,

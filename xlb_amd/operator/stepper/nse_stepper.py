@@ -3,7 +3,7 @@ stepper.py:6-34).
 
 ``__call__(f_0, f_1, bc_mask, missing_mask, omega, timestep)`` runs ONE fused HIP kernel
 (libxlbhip ``xlbh_step``): pull streaming, streaming-step BCs, macroscopic, equilibrium,
-BGK/KBC collision (+ exact-difference force), collision-step BCs, extrapolation-outflow update
+BGK/KBC/Smagorinsky collision (+ exact-difference force), collision-step BCs, extrapolation-outflow update
 and store, writing f_1 and returning (f_0, f_1); the caller swaps, as with the reference.
 
 Per-mask device state (derived once and re-derived only when the mask tensors change):
@@ -232,8 +232,8 @@ class IncompressibleNavierStokesStepper(Stepper):
     )
 
     def _check_adjoint_supported(self):
-        if self.collision_type != "BGK":
-            raise NotImplementedError("the adjoint step is implemented for BGK collision")
+        if self.collision_type not in ("BGK", "SmagorinskyLESBGK"):
+            raise NotImplementedError("the adjoint step is implemented for the BGK and SmagorinskyLESBGK collisions")
 
     def _adjoint_cells(self):
         """(pointer, count) of the cells the adjoint's boundary pass runs over: every cell whose BC
@@ -253,7 +253,7 @@ class IncompressibleNavierStokesStepper(Stepper):
     def adjoint_step(self, f_t, lam_out, lam_in, bc_mask, missing_mask, omega, grad_force=None):
         """Reverse mode of one step (xlbh_adjoint_step): given the state f_t a forward step read and
         lam_out = dL/df_{t+1}, write lam_in = dL/df_t and add dL/dg to ``grad_force`` ((d, nx, ny, nz)
-        float64) when the stepper has a body force.  BGK only."""
+        float64) when the stepper has a body force.  BGK and SmagorinskyLESBGK collisions."""
         self._check_adjoint_supported()
         nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_t, None, bc_mask, missing_mask)
         q, ext = self.velocity_set.q, (nx, ny, nz)
@@ -267,10 +267,10 @@ class IncompressibleNavierStokesStepper(Stepper):
             raise NotImplementedError("the adjoint step does not support solid (255) cells")
         _lib.check(
             _lib.lib().xlbh_adjoint_step(
-                self.lattice_id, self.precision_policy.store_format, self.compute_id,
+                self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_t), _lib.ptr(lam_out), _lib.ptr(lam_in), *masks, *self._adjoint_cells(), force_mode, fu_ptr,
                 _lib.ptr(force_field),
-                _lib.ptr(grad_force), nx, ny, nz, float(omega), _lib.stream_of(f_t),
+                _lib.ptr(grad_force), nx, ny, nz, float(omega), self._collision_param(), _lib.stream_of(f_t),
             ),
             "IncompressibleNavierStokesStepper.adjoint_step",
         )
@@ -313,7 +313,8 @@ class IncompressibleNavierStokesStepper(Stepper):
         segment end.  Returns the buffer (lam_a or lam_b) that holds dL/df_ckpt.  With
         ``recompute=False`` the states are taken as recorded by ``record``.  A ``workspace`` (from
         ``adjoint_workspace``) selects the fused one-kernel reverse step; a sweep over several
-        segments shares it, with ``begin`` on its first call and ``end`` on its last.  BGK only."""
+        segments shares it, with ``begin`` on its first call and ``end`` on its last.  BGK and
+        SmagorinskyLESBGK collisions."""
         self._check_adjoint_supported()
         if self._has_solid:
             raise NotImplementedError("the adjoint step does not support solid (255) cells")
@@ -339,10 +340,11 @@ class IncompressibleNavierStokesStepper(Stepper):
             flags |= (_lib.ADJ_BEGIN if begin else 0) | (_lib.ADJ_END if end else 0)
         _lib.check(
             _lib.lib().xlbh_adjoint_run(
-                self.lattice_id, self.precision_policy.store_format, self.compute_id,
+                self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_ckpt), _lib.ptr(states), _lib.ptr(lam_a), _lib.ptr(lam_b), *masks, *self._adjoint_cells(),
                 force_mode, fu_ptr, _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, float(omega),
-                int(timestep), int(num_steps), _lib.ptr(workspace), flags, _lib.stream_of(f_ckpt),
+                self._collision_param(), int(timestep), int(num_steps), _lib.ptr(workspace), flags,
+                _lib.stream_of(f_ckpt),
             ),
             "IncompressibleNavierStokesStepper.adjoint_run",
         )
