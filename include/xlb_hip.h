@@ -261,8 +261,8 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
                      int nx, int ny, int nz, void* stream);
 
 /* ---------------------------------------------------------------------------------------
- * Reverse mode of the fused step (BGK and Smagorinsky BGK, every boundary kind; `collision` and
- * `collision_param` as for xlbh_step, KBC is rejected).  No counterpart function in the
+ * Reverse mode of the fused step (every collision and boundary kind; `collision` and
+ * `collision_param` as for xlbh_step, KBC on D3Q19 rejected).  No counterpart function in the
  * reference, whose gradients come from Warp's tape replaying generated adjoint kernels
  * (examples/out_of_core/subroutine/autodiff_stepper_subroutine.py:459-466).
  *

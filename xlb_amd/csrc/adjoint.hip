@@ -1,4 +1,4 @@
-// Reverse mode of the fused step (BGK, and Smagorinsky BGK through collide_adjoint): given the
+// Reverse mode of the fused step (BGK; KBC and Smagorinsky BGK through collide_adjoint): given the
 // state f^t that a forward step read and
 // lam_out = dL/df^{t+1}, produce lam_in = dL/df^t and accumulate dL/dg for a per-cell body force.
 //
@@ -86,16 +86,113 @@ __device__ __forceinline__ void eq_moments(const C* lam, const C* u, const C* v,
 // >= 2): the BGK adjoint with k for omega plus the strain term
 //   -(sum_l G_l fneq_l) dk/dS dS/dh_m,  dS/dh_m = 2 a_m fneq_m - (A + B.(c_m - u))[2 a fneq],
 //   dk/dS = -k^2 9 Cs^2 / (2 sqrt(tau0^2 + 36 Cs^2 sqrt S) sqrt S)  (taken as 0 at S = 0).
+// KBC (physics.hpp, kbc.py:192-296): out = h - beta (2 ds + gamma dh), fneq = h - feq(rho, u),
+// Pi = cc.fneq, ds = rho s(Pi) (s / 4 on D2Q9), dh = fneq - ds,
+// gamma = 1/beta - (2 - 1/beta) sp1 / (eps + sp2), sp1 = sum dh ds / feq, sp2 = sum dh^2 / feq.
+// Reverse in closed form: through gamma (sp1, sp2 and their 1/feq weights), the shear map's
+// transpose (s is linear in Pi), rho (ds = rho s and dRho/dh = 1) and feq(rho, u) via the
+// equilibrium moments A + B.(c_m - u).
 template <int LAT, class C>
-__device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, bool has_force, const C* F, C* gg,
-                                                bool smag = false, C smag_cs = C(0)) {
+__device__ __forceinline__ void kbc_adjoint(const C* h, C* lam, C omega, bool has_force, const C* F, C* gg) {
   constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  C rho, u[3];
+  macroscopic<LAT>(h, rho, u);
+  const C usqr = usqr15<LAT>(u);
+  C pi[6];
+  pi_neq<LAT>(h, rho, u, usqr, pi);
+  const C sc = (T.d == 2) ? C(0.25) : C(1);
+  const C beta = C(0.5) * omega;
+  const C inv_beta = C(1) / beta;
+  C fe[Q], sh[Q], ds[Q], dh[Q];
+  C sp1 = C(0), sp2 = C(0);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    fe[l] = feq_l<LAT>(l, rho, u, usqr);
+    sh[l] = kbc_shear<LAT>(l, pi);
+    ds[l] = sh[l] * rho * sc;
+    dh[l] = (h[l] - fe[l]) - ds[l];
+    const C tmp = dh[l] / fe[l];
+    sp1 += tmp * ds[l];
+    sp2 += tmp * dh[l];
+  }
+  const C D = C(1e-32) + sp2;
+  const C c2 = C(2) - inv_beta;
+  const C gamma = inv_beta - c2 * sp1 / D;
+  C Gg = C(0);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) Gg += lam[l] * dh[l];
+  Gg *= -beta;
+  const C Gsp1 = Gg * (-c2 / D);
+  const C Gsp2 = Gg * (c2 * sp1 / (D * D));
+  C Gfe[Q], Gfn[Q], Gs[Q];
+  C Grho = C(0);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    const C ife = C(1) / fe[l];
+    C Gds = -C(2) * beta * lam[l] + Gsp1 * dh[l] * ife;
+    const C Gdh = -beta * gamma * lam[l] + (Gsp1 * ds[l] + C(2) * Gsp2 * dh[l]) * ife;
+    Gfe[l] = -(Gsp1 * dh[l] * ds[l] + Gsp2 * dh[l] * dh[l]) * ife * ife;
+    Gfn[l] = Gdh;
+    Gds -= Gdh;
+    Gs[l] = Gds * rho * sc;
+    Grho += Gds * sh[l] * sc;
+  }
+  // s is linear in Pi: dL/dPi_t = sum_l Gs_l s_l(e_t)
+  C GP[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+#pragma unroll
+  for (int t = 0; t < T.nt; ++t) {
+    C e[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+    e[t] = C(1);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) GP[t] += Gs[l] * kbc_shear<LAT>(l, e);
+  }
+#pragma unroll
+  for (int m = 0; m < Q; ++m) {
+#pragma unroll
+    for (int t = 0; t < T.nt; ++t) {
+      if (T.cc[m][t] == 1) Gfn[m] += GP[t];
+      else if (T.cc[m][t] == -1) Gfn[m] -= GP[t];
+    }
+    Gfe[m] -= Gfn[m];  // fneq = h - feq
+  }
+  C A1 = C(0), B1[3] = {C(0), C(0), C(0)};
+  if (has_force) {
+    C ug[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ug[d] = u[d] + F[d];
+    eq_moments<LAT>(lam, u, ug, A1, B1);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) Gfe[l] -= lam[l];  // the - feq(rho, u) of the exact difference
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gg[d] = has_force ? rho * B1[d] : C(0);
+  C A0, B0[3];
+  eq_moments<LAT>(Gfe, u, u, A0, B0);
+#pragma unroll
+  for (int m = 0; m < Q; ++m) {
+    C s = lam[m] + Gfn[m] + Grho + A0 + A1;
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) s += (B0[d] + B1[d]) * (C(T.c[d][m]) - u[d]);
+    lam[m] = s;
+  }
+}
+
+template <int LAT, int COLL, class C>
+__device__ __forceinline__ void collide_adjoint(const C* h, C* lam, C omega, bool has_force, const C* F, C* gg,
+                                                C smag_cs = C(0)) {
+  constexpr LatTables T = make_tables(LAT);
+  if constexpr (COLL == XLBH_KBC) {
+    kbc_adjoint<LAT>(h, lam, omega, has_force, F, gg);
+    return;
+  }
+  constexpr bool smag = COLL == XLBH_SMAGORINSKY_BGK;
   C rho, u[3];
   macroscopic<LAT>(h, rho, u);
   C strain[T.q];
 #pragma unroll
   for (int k = 0; k < T.q; ++k) strain[k] = C(0);
-  if (smag) {
+  if constexpr (smag) {
     const C usqr = usqr15<LAT>(u);
     C fneq[T.q], lamS[T.q];
     C S = C(0), Gf = C(0);
@@ -398,7 +495,7 @@ __device__ __forceinline__ void own_pre(const AdjParams& p, uint32_t cell, C* pr
 }
 
 // Kernel A: every cell pushes G_ps into lam_in (a permutation) and adds its force gradient.
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
@@ -435,7 +532,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
     if (kind == XLBH_BC_EQUILIBRIUM && p.force_mode != XLBH_FORCE_NONE && p.grad_force) {
       C F[3], gg[3];
       cell_force<LAT>(p, cell, F);
-      collide_adjoint<LAT>(h, lam, (C)p.omega, true, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
+      collide_adjoint<LAT, COLL>(h, lam, (C)p.omega, true, F, gg, (C)p.cparam);
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
     }
@@ -445,7 +542,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
     const bool has_force = p.force_mode != XLBH_FORCE_NONE;
 #pragma unroll
     for (int k = 0; k < Q; ++k) G[k] = lam[k];
-    collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
+    collide_adjoint<LAT, COLL>(h, G, (C)p.omega, has_force, F, gg, (C)p.cparam);
     if (has_force && p.grad_force) {
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
@@ -467,7 +564,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
 // Boundary-cell contributions that do not follow the pull, for one boundary cell.  `part` selects
 // the work: -1 everything; 0 everything except the Zou-He / regularized Jacobian; 1 + k the
 // Jacobian column of input direction k only (so a cell's columns can run on separate threads).
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cell, int x, int y, int z, int part) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
@@ -492,7 +589,7 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
   C G[Q];
 #pragma unroll
   for (int k = 0; k < Q; ++k) G[k] = lam[k];
-  collide_adjoint<LAT>(h, G, (C)p.omega, has_force, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
+  collide_adjoint<LAT, COLL>(h, G, (C)p.omega, has_force, F, gg, (C)p.cparam);
 
   if (part <= 0) {
     if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
@@ -564,17 +661,17 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
 }
 
 // Kernel B over the whole grid (no boundary-cell list given).
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 __global__ __launch_bounds__(256) void k_adjoint_bc(AdjParams p) {
   int x, y, z;
   if (!sweep_cell(p.sw, x, y, z)) return;
   const uint32_t cell = ((uint32_t)x * p.sw.ny + y) * p.sw.nz + z;
-  adjoint_bc_cell<LAT, FMT, C>(p, cell, x, y, z, -1);
+  adjoint_bc_cell<LAT, FMT, C, COLL>(p, cell, x, y, z, -1);
 }
 
 // Kernel B over a list of boundary cells: Q + 1 threads per cell (part 0 and the Q Jacobian
 // columns).
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 __global__ __launch_bounds__(256) void k_adjoint_bc_list(AdjParams p) {
   constexpr int Q = make_tables(LAT).q;
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -587,7 +684,7 @@ __global__ __launch_bounds__(256) void k_adjoint_bc_list(AdjParams p) {
   const int z = (int)(cell % (uint32_t)nz);
   const int y = (int)((cell / (uint32_t)nz) % (uint32_t)ny);
   const int x = (int)(cell / ((uint32_t)nz * (uint32_t)ny));
-  adjoint_bc_cell<LAT, FMT, C>(p, cell, x, y, z, part);
+  adjoint_bc_cell<LAT, FMT, C, COLL>(p, cell, x, y, z, part);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -604,7 +701,7 @@ __device__ __forceinline__ bool own_kind(int kind) {
          kind == XLBH_BC_GRADS_APPROXIMATION;
 }
 
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
@@ -663,7 +760,7 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
     const bool has_force = p.force_mode != XLBH_FORCE_NONE;
 #pragma unroll
     for (int k = 0; k < Q; ++k) Gh[k] = lam[k];
-    collide_adjoint<LAT>(h, Gh, (C)p.omega, has_force, F, gg, p.collision == XLBH_SMAGORINSKY_BGK, (C)p.cparam);
+    collide_adjoint<LAT, COLL>(h, Gh, (C)p.omega, has_force, F, gg, (C)p.cparam);
     if (has_force && p.grad_force) {
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) p.grad_force[(size_t)dd * p.N + cell] += (double)gg[dd];
@@ -783,15 +880,15 @@ __global__ __launch_bounds__(256) void k_zero_words(uint32_t* p, uint64_t n) {
 
 enum AdjMode { ADJ_SPLIT = 0, ADJ_FUSED = 1, ADJ_BEGIN = 2, ADJ_FLUSH = 3 };
 
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 int launch_adjoint(const AdjParams& p, hipStream_t s);
 
 // The fused path's kernels.  `ws_words` / `flag` / `lam` are used by BEGIN / FLUSH.
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 int launch_adjoint_mode(const AdjParams& p, hipStream_t s, int mode, uint64_t ws_words, uint8_t* flag, void* lam) {
-  if (mode == ADJ_SPLIT) return launch_adjoint<LAT, FMT, C>(p, s);
+  if (mode == ADJ_SPLIT) return launch_adjoint<LAT, FMT, C, COLL>(p, s);
   if (mode == ADJ_FUSED) {
-    hipLaunchKernelGGL((k_adjoint_fused<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((k_adjoint_fused<LAT, FMT, C, COLL>), sweep_grid(p.sw), dim3(256), 0, s, p);
     XLBH_LAUNCH_CHECK();
     return 0;
   }
@@ -811,19 +908,19 @@ int launch_adjoint_mode(const AdjParams& p, hipStream_t s, int mode, uint64_t ws
   return 0;
 }
 
-template <int LAT, int FMT, class C>
+template <int LAT, int FMT, class C, int COLL>
 int launch_adjoint(const AdjParams& p, hipStream_t s) {
-  hipLaunchKernelGGL((k_adjoint_push<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((k_adjoint_push<LAT, FMT, C, COLL>), sweep_grid(p.sw), dim3(256), 0, s, p);
   XLBH_LAUNCH_CHECK();
   if (p.bc_mask && p.bc_cells) {
     if (p.n_bc_cells > 0) {
       constexpr int Q = make_tables(LAT).q;
       const int64_t threads = (int64_t)p.n_bc_cells * (Q + 1);
-      hipLaunchKernelGGL((k_adjoint_bc_list<LAT, FMT, C>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, p);
+      hipLaunchKernelGGL((k_adjoint_bc_list<LAT, FMT, C, COLL>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, p);
       XLBH_LAUNCH_CHECK();
     }
   } else if (p.bc_mask) {
-    hipLaunchKernelGGL((k_adjoint_bc<LAT, FMT, C>), sweep_grid(p.sw), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((k_adjoint_bc<LAT, FMT, C, COLL>), sweep_grid(p.sw), dim3(256), 0, s, p);
     XLBH_LAUNCH_CHECK();
   }
   return 0;
@@ -836,25 +933,39 @@ using namespace xlbh;
 static int adjoint_dispatch(int lattice, int store, int compute, const AdjParams& p, hipStream_t s,
                             int mode = ADJ_SPLIT, uint64_t ws_words = 0, uint8_t* flag = nullptr,
                             void* lam = nullptr) {
-  auto go = [&](auto L) -> int {
+  // the collision is a template parameter, so each reverse kernel carries only its own collision
+  auto with_coll = [&](auto L, auto K) -> int {
     constexpr int LAT = decltype(L)::value;
+    constexpr int COLL = decltype(K)::value;
     if (compute == XLBH_COMPUTE_F32) {
       switch (store) {
-        case XLBH_F32: return launch_adjoint_mode<LAT, XLBH_F32, float>(p, s, mode, ws_words, flag, lam);
-        case XLBH_F16: return launch_adjoint_mode<LAT, XLBH_F16, float>(p, s, mode, ws_words, flag, lam);
-        case XLBH_BF16: return launch_adjoint_mode<LAT, XLBH_BF16, float>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F32: return launch_adjoint_mode<LAT, XLBH_F32, float, COLL>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F16: return launch_adjoint_mode<LAT, XLBH_F16, float, COLL>(p, s, mode, ws_words, flag, lam);
+        case XLBH_BF16: return launch_adjoint_mode<LAT, XLBH_BF16, float, COLL>(p, s, mode, ws_words, flag, lam);
         case XLBH_F16_SHIFTED:
-          return launch_adjoint_mode<LAT, XLBH_F16_SHIFTED, float>(p, s, mode, ws_words, flag, lam);
+          return launch_adjoint_mode<LAT, XLBH_F16_SHIFTED, float, COLL>(p, s, mode, ws_words, flag, lam);
         case XLBH_BF16_SHIFTED:
-          return launch_adjoint_mode<LAT, XLBH_BF16_SHIFTED, float>(p, s, mode, ws_words, flag, lam);
+          return launch_adjoint_mode<LAT, XLBH_BF16_SHIFTED, float, COLL>(p, s, mode, ws_words, flag, lam);
       }
     } else if (compute == XLBH_COMPUTE_F64) {
       switch (store) {
-        case XLBH_F64: return launch_adjoint_mode<LAT, XLBH_F64, double>(p, s, mode, ws_words, flag, lam);
-        case XLBH_F32: return launch_adjoint_mode<LAT, XLBH_F32, double>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F64: return launch_adjoint_mode<LAT, XLBH_F64, double, COLL>(p, s, mode, ws_words, flag, lam);
+        case XLBH_F32: return launch_adjoint_mode<LAT, XLBH_F32, double, COLL>(p, s, mode, ws_words, flag, lam);
       }
     }
     set_error("unsupported (store, compute) combination for the adjoint step");
+    return 1;
+  };
+  auto go = [&](auto L) -> int {
+    constexpr int LAT = decltype(L)::value;
+    // BEGIN / FLUSH do not depend on the collision
+    if (mode == ADJ_BEGIN || mode == ADJ_FLUSH || p.collision == XLBH_BGK)
+      return with_coll(L, std::integral_constant<int, XLBH_BGK>{});
+    if (p.collision == XLBH_SMAGORINSKY_BGK) return with_coll(L, std::integral_constant<int, XLBH_SMAGORINSKY_BGK>{});
+    if constexpr (LAT != XLBH_D3Q19) {
+      if (p.collision == XLBH_KBC) return with_coll(L, std::integral_constant<int, XLBH_KBC>{});
+    }
+    set_error("unsupported collision for the adjoint step");
     return 1;
   };
   switch (lattice) {
@@ -870,8 +981,10 @@ static int adjoint_validate(int lattice, int collision, int force_mode, const do
                             const void* force_field, const uint8_t* bc_mask, const uint32_t* missing_bits,
                             const xlbh_bc_desc* bc_table, int nx, int ny, int nz) {
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
-  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_SMAGORINSKY_BGK,
-               "the adjoint supports BGK and Smagorinsky BGK collisions");
+  XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC || collision == XLBH_SMAGORINSKY_BGK,
+               "unknown collision");
+  XLBH_REQUIRE(!(collision == XLBH_KBC && lattice == XLBH_D3Q19),
+               "KBC collision is defined for D2Q9 and D3Q27 only (kbc.py:183-185)");
   XLBH_REQUIRE(!bc_mask || (missing_bits && bc_table), "bc_mask given without missing bits or BC table");
   XLBH_REQUIRE(force_mode >= XLBH_FORCE_NONE && force_mode <= XLBH_FORCE_FIELD, "unknown force mode");
   XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");

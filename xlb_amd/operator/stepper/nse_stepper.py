@@ -232,8 +232,8 @@ class IncompressibleNavierStokesStepper(Stepper):
     )
 
     def _check_adjoint_supported(self):
-        if self.collision_type not in ("BGK", "SmagorinskyLESBGK"):
-            raise NotImplementedError("the adjoint step is implemented for the BGK and SmagorinskyLESBGK collisions")
+        if self.collision_type not in ("BGK", "KBC", "SmagorinskyLESBGK"):
+            raise NotImplementedError(f"no adjoint for collision {self.collision_type}")
 
     def _adjoint_cells(self):
         """(pointer, count) of the cells the adjoint's boundary pass runs over: every cell whose BC
@@ -253,7 +253,7 @@ class IncompressibleNavierStokesStepper(Stepper):
     def adjoint_step(self, f_t, lam_out, lam_in, bc_mask, missing_mask, omega, grad_force=None):
         """Reverse mode of one step (xlbh_adjoint_step): given the state f_t a forward step read and
         lam_out = dL/df_{t+1}, write lam_in = dL/df_t and add dL/dg to ``grad_force`` ((d, nx, ny, nz)
-        float64) when the stepper has a body force.  BGK and SmagorinskyLESBGK collisions."""
+        float64) when the stepper has a body force.  Every collision (BGK, KBC, SmagorinskyLESBGK)."""
         self._check_adjoint_supported()
         nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_t, None, bc_mask, missing_mask)
         q, ext = self.velocity_set.q, (nx, ny, nz)
@@ -313,8 +313,8 @@ class IncompressibleNavierStokesStepper(Stepper):
         segment end.  Returns the buffer (lam_a or lam_b) that holds dL/df_ckpt.  With
         ``recompute=False`` the states are taken as recorded by ``record``.  A ``workspace`` (from
         ``adjoint_workspace``) selects the fused one-kernel reverse step; a sweep over several
-        segments shares it, with ``begin`` on its first call and ``end`` on its last.  BGK and
-        SmagorinskyLESBGK collisions."""
+        segments shares it, with ``begin`` on its first call and ``end`` on its last.  Every
+        collision (BGK, KBC, SmagorinskyLESBGK)."""
         self._check_adjoint_supported()
         if self._has_solid:
             raise NotImplementedError("the adjoint step does not support solid (255) cells")
