@@ -205,143 +205,6 @@ __global__ __launch_bounds__(256) void k_step(StepParams p) {
 }
 
 
-// ------------------------------------------------------------------------------------------
-// Direction-split step (BGK).  A block of W waves updates 64 consecutive cells; wave v pulls
-// and stores only the directions l = v, v + W, v + 2W, ...  The pulled populations meet in
-// LDS (Q x 64 values), so after one barrier every wave sees the whole post-stream state of its
-// 64 cells.  Each wave then carries ceil(Q/W) loads and stores instead of Q: on gfx950 the
-// sustained HBM rate of a sweep falls as the per-thread load/store count grows, and this
-// brings it to the level of a 4-5 stream copy.  Fluid cells compute rho and u from LDS and
-// update only the wave's own directions; boundary cells run the full per-cell update
-// (redundantly in each wave) and each wave stores its own directions of the result.
-// ------------------------------------------------------------------------------------------
-template <int LAT, int FMT, class C, class O, int V, int W>
-__device__ __forceinline__ void split_pull(C (*sf)[64], int lane, const PullOffsets<O>& po,
-                                           const Planes<typename Fmt<FMT>::raw, O>& fin) {
-  constexpr LatTables T = make_tables(LAT);
-#pragma unroll
-  for (int l = V; l < T.q; l += W) sf[l][lane] = ld_pop<FMT, C>(fin.at(l, po.template src<LAT>(l)), T.w[l]);
-}
-
-template <int LAT, int FMT, class C, class O, int V, int W>
-__device__ __forceinline__ void split_store_fluid(C (*sf)[64], int lane, O cell, const StepParams& p,
-                                                  const Planes<typename Fmt<FMT>::raw, O>& fout) {
-  constexpr LatTables T = make_tables(LAT);
-  constexpr int Q = T.q;
-  C f[Q];
-#pragma unroll
-  for (int l = 0; l < Q; ++l) f[l] = sf[l][lane];
-  C rho, u[3];
-  macroscopic<LAT>(f, rho, u);
-  const C usqr = usqr15<LAT>(u);
-  const C omega = (C)p.omega;
-  C F[3] = {C(0), C(0), C(0)};
-  const bool has_force = p.force_mode != XLBH_FORCE_NONE;
-  if (p.force_mode == XLBH_FORCE_UNIFORM) {
-    F[0] = (C)p.fu[0];
-    F[1] = (C)p.fu[1];
-    F[2] = (C)p.fu[2];
-  } else if (p.force_mode == XLBH_FORCE_FIELD) {
-    const C* ff = static_cast<const C*>(p.force_field);
-#pragma unroll
-    for (int dd = 0; dd < T.d; ++dd) F[dd] = ff[(size_t)dd * p.N + cell];
-  }
-  C uf[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) uf[d] = u[d] + F[d];
-  const C usqr_f = usqr15<LAT>(uf);
-#pragma unroll
-  for (int l = V; l < Q; l += W) {
-    const C fe = feq_l<LAT>(l, rho, u, usqr);
-    C out = f[l] - omega * (f[l] - fe);
-    if (has_force) out += feq_l<LAT>(l, rho, uf, usqr_f) - fe;
-    st_pop<FMT, C>(fout.at_w(l, cell), out, T.w[l]);
-  }
-}
-
-template <int LAT, int FMT, class C, class O, int V, int W>
-__device__ __forceinline__ void split_store_boundary(C (*sf)[64], int lane, int id, O cell, int x, int y, int z,
-                                                     const StepParams& p, const Planes<typename Fmt<FMT>::raw, O>& fin,
-                                                     const Planes<typename Fmt<FMT>::raw, O>& fout) {
-  constexpr LatTables T = make_tables(LAT);
-  constexpr int Q = T.q;
-  C f[Q];
-#pragma unroll
-  for (int l = 0; l < Q; ++l) f[l] = sf[l][lane];
-  cell_update<LAT, XLBH_BGK, FMT, C, O>(f, id, cell, x, y, z, p, fin);
-#pragma unroll
-  for (int l = V; l < Q; l += W) st_pop<FMT, C>(fout.at_w(l, cell), f[l], T.w[l]);
-}
-
-#ifndef XLBH_SPLIT_WAVES
-#define XLBH_SPLIT_WAVES 0
-#endif
-#if XLBH_SPLIT_WAVES > 0
-#define XLBH_SPLIT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(XLBH_SPLIT_WAVES, XLBH_SPLIT_WAVES)))
-#else
-#define XLBH_SPLIT_OCCUPANCY
-#endif
-
-template <int LAT, int FMT, class C, bool WIDE, int W>
-__global__ __launch_bounds__(64 * W) XLBH_SPLIT_OCCUPANCY void k_step_split(StepParams p) {
-  constexpr LatTables T = make_tables(LAT);
-  constexpr int Q = T.q;
-  using R = typename Fmt<FMT>::raw;
-  using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
-  __shared__ C sf[Q][64];
-
-  const int lane = threadIdx.x & 63;
-  const int v = threadIdx.x >> 6;  // wave index: uniform per wave
-  const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
-  const uint64_t c64 = (uint64_t)blockIdx.x * 64 + lane;
-  const bool valid = c64 < (uint64_t)p.N;
-  const O cell = valid ? (O)c64 : (O)0;
-  const O t = cell / (O)nz;
-  const int z = (int)(cell - t * (O)nz);
-  const int y = (int)(t % (O)ny);
-  const int x = (int)(t / (O)ny);
-  const size_t plane_bytes = (size_t)p.N * sizeof(R);
-  const Planes<R, O> fin{static_cast<const char*>(p.f_in), plane_bytes};
-  const Planes<R, O> fout{static_cast<const char*>(p.f_out), plane_bytes};
-  const PullOffsets<O> po(x, y, z, nx, ny, nz);
-
-  // Lanes past the end pull cell 0's sources (in bounds) and store nothing.
-  switch (v) {
-    case 0: split_pull<LAT, FMT, C, O, 0, W>(sf, lane, po, fin); break;
-    case 1: split_pull<LAT, FMT, C, O, 1 % W, W>(sf, lane, po, fin); break;
-    case 2: split_pull<LAT, FMT, C, O, 2 % W, W>(sf, lane, po, fin); break;
-    case 3: split_pull<LAT, FMT, C, O, 3 % W, W>(sf, lane, po, fin); break;
-    default: split_pull<LAT, FMT, C, O, 4 % W, W>(sf, lane, po, fin); break;
-  }
-  int id = 0;
-  if (p.bc_mask != nullptr && valid) id = p.bc_mask[cell];
-  __syncthreads();
-  if (!valid || id == 255) return;  // solid: no update (nse_stepper.py:356-358)
-
-  if (id == 0) {
-    switch (v) {
-      case 0: split_store_fluid<LAT, FMT, C, O, 0, W>(sf, lane, cell, p, fout); break;
-      case 1: split_store_fluid<LAT, FMT, C, O, 1 % W, W>(sf, lane, cell, p, fout); break;
-      case 2: split_store_fluid<LAT, FMT, C, O, 2 % W, W>(sf, lane, cell, p, fout); break;
-      case 3: split_store_fluid<LAT, FMT, C, O, 3 % W, W>(sf, lane, cell, p, fout); break;
-      default: split_store_fluid<LAT, FMT, C, O, 4 % W, W>(sf, lane, cell, p, fout); break;
-    }
-  } else {
-    switch (v) {
-      case 0: split_store_boundary<LAT, FMT, C, O, 0, W>(sf, lane, id, cell, x, y, z, p, fin, fout); break;
-      case 1: split_store_boundary<LAT, FMT, C, O, 1 % W, W>(sf, lane, id, cell, x, y, z, p, fin, fout); break;
-      case 2: split_store_boundary<LAT, FMT, C, O, 2 % W, W>(sf, lane, id, cell, x, y, z, p, fin, fout); break;
-      case 3: split_store_boundary<LAT, FMT, C, O, 3 % W, W>(sf, lane, id, cell, x, y, z, p, fin, fout); break;
-      default: split_store_boundary<LAT, FMT, C, O, 4 % W, W>(sf, lane, id, cell, x, y, z, p, fin, fout); break;
-    }
-  }
-}
-
-// Waves per block of the direction-split step (0 = use the one-thread-per-cell kernel).
-#ifndef XLBH_SPLIT_W
-#define XLBH_SPLIT_W 0
-#endif
-
 // Host launcher, instantiated per lattice in step_<lattice>.hip.
 struct StepCall {
   int collision, store, compute;
@@ -377,20 +240,11 @@ int launch_step_t(const StepCall& c) {
   p.sw = make_sweep(c.nx, c.ny, c.nz);
   p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
   const bool wide = (uint64_t)p.N * sizeof(typename Fmt<FMT>::raw) > 0xffffffffull;
-  constexpr int W = XLBH_SPLIT_W;
-  if constexpr (COLL == XLBH_BGK && W > 0) {
-    const dim3 grid((unsigned)(((uint64_t)p.N + 63) / 64));
-    if (wide)
-      hipLaunchKernelGGL((k_step_split<LAT, FMT, C, true, W>), grid, dim3(64 * W), 0, c.stream, p);
-    else
-      hipLaunchKernelGGL((k_step_split<LAT, FMT, C, false, W>), grid, dim3(64 * W), 0, c.stream, p);
-  } else {
-    const unsigned lds = sizeof(typename Fmt<FMT>::raw) >= 4 ? kStepLdsCap : 0u;
-    if (wide)
-      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
-    else
-      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
-  }
+  const unsigned lds = sizeof(typename Fmt<FMT>::raw) >= 4 ? kStepLdsCap : 0u;
+  if (wide)
+    hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+  else
+    hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
   XLBH_LAUNCH_CHECK();
   return 0;
 }
