@@ -42,9 +42,13 @@ class MeshBoundaryMasker(Operator):
         expect_field(bc_mask, 1, (nx, ny, nz), "bc_mask", torch.uint8)
         expect_field(missing_mask, self.velocity_set.q, (nx, ny, nz), "missing_mask", torch.bool)
         domain_shape = (nx, ny, nz)
+        # the vertices go to the GPU as they are; the extent check (exact in any float type) and the
+        # cast to float32 run there instead of on the host
+        verts = torch.from_numpy(np.ascontiguousarray(mesh_vertices)).to(bc_mask.device)
         if mesh_vertices.shape[0]:
-            mesh_min = np.min(mesh_vertices, axis=0)
-            mesh_max = np.max(mesh_vertices, axis=0)
+            lo, hi = torch.aminmax(verts, dim=0)
+            ext = torch.stack([lo, hi]).cpu().numpy()
+            mesh_min, mesh_max = ext[0], ext[1]
             if any(mesh_min < 0) or any(mesh_max >= domain_shape):
                 raise ValueError(
                     f"Mesh extents ({mesh_min}, {mesh_max}) exceed domain dimensions {domain_shape}. "
@@ -59,7 +63,8 @@ class MeshBoundaryMasker(Operator):
         nbytes = int(L.xlbh_mesh_workspace_bytes(nx, ny, nz))
         if self._ws is None or self._ws.numel() < nbytes or self._ws.device != bc_mask.device:
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=bc_mask.device)
-        tri = torch.from_numpy(np.ascontiguousarray(mesh_vertices, dtype=np.float32).reshape(-1, 9)).to(bc_mask.device)
+        tri = verts.to(torch.float32).reshape(-1, 9).contiguous()
+        del verts
         _lib.check(
             L.xlbh_mesh_mask(self.lattice_id, _lib.ptr(tri), int(tri.shape[0]), int(bc.id), _lib.ptr(bc_mask),
                              _lib.ptr(missing_mask), _lib.ptr(self._ws), nx, ny, nz, _lib.stream_of(bc_mask)),
