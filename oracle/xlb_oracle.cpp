@@ -12,6 +12,13 @@
 // takes the "all reads before writes" order.  Citations are paths relative to the reference
 // repository root.
 //
+// Every function is pinned by tests/test_oracle_cpu.py, tests/test_mesh_cpu.py and the golden
+// fixtures (known-answer tests of the reference's own test suite, lattice identities,
+// conservation, closed-form flows and special cases).  One aspect is parity unpinned: bit-level
+// agreement of the triangle / box test (orc_tri_box) with the reference's Warp formulation of it,
+// whose matrix-from-vectors convention cannot be run here.  The test itself is pinned against an
+// independent float64 separating-axis test and a closed-form box voxelisation.
+//
 // Build: see oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
 #include <cmath>
 #include <cstdint>
