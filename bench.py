@@ -16,9 +16,9 @@ Also reported on the same line:
 * adjoint -- backward (checkpoint recompute + adjoint kernels) over forward time per step for the
   differentiable D2Q9 channel 512x256 (800 steps, checkpoint every 50), the 256^3 cavity and the
   128^3 decaying turbulence (2000 steps, BF16 checkpoints every 45);
-* kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, with the momentum-exchange drag
-  every 100 steps inside the timed loop; grads_sphere -- the same with a GradsApproximationBC
-  sphere;
+* kbc_sphere -- D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, 5000 steps with the
+  momentum-exchange drag every 100 steps inside the timed loop, and the drag coefficient at the
+  end; grads_sphere -- the same with a GradsApproximationBC sphere;
 * mesh_masker -- surface voxelisation of an 81,920-triangle sphere mesh into the same grid;
 * les_cavity -- the LES lid-driven cavity of examples/out_of_core/lid_driven_cavity.py (D3Q19
   SmagorinskyLESBGK, Cs = 0.17, 256^3, lid u = 0.06, tau = 0.501, FP32), with the Q-criterion
@@ -327,13 +327,15 @@ def turbulence_adjoint(xa, torch, quick=False):
     return r
 
 
-def kbc_sphere_leg(xa, torch, steps=500, quick=False, sphere_bc="halfway"):
+def kbc_sphere_leg(xa, torch, steps=5000, quick=False, sphere_bc="halfway"):
     """D3Q27 KBC wind tunnel with the drag every 100 steps inside the timed loop; the sphere is a
     halfway bounce-back or a Grad's-approximation wall."""
     from xlb_amd.workloads import wind_tunnel_sphere
 
     shape = (128, 64, 64) if quick else (256, 128, 128)
     radius, center = (6, (32, 32, 32)) if quick else (12, (64, 64, 64))
+    if quick:
+        steps = min(steps, 500)
     wl = wind_tunnel_sphere(shape, xa.PrecisionPolicy.FP32FP32, radius, center, sphere_bc=sphere_bc)
     st = wl.stepper
     mt = xa.operator.force.MomentumTransfer(wl.extra["sphere_bc"])
