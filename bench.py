@@ -157,6 +157,36 @@ def precision_leg(xa, torch, shape=(512, 256, 256), steps=2000):
                          f"nu=0.01, {steps} steps; errors against the FP64/FP64 run", formats=out)
 
 
+def precision_viscosity_sweep(xa, torch, shape=(512, 256, 256), steps=2000, nus=(0.003, 0.03, 0.1)):
+    """The C3 Taylor-Green vortex at other viscosities: relative L2 velocity error of the shifted
+    16-bit formats against FP64, to show how the storage error depends on the relaxation time."""
+    from xlb_amd.workloads import taylor_green_3d
+
+    P = xa.PrecisionPolicy
+    res = {}
+    for nu in nus:
+        row = {"tau": 3 * nu + 0.5}
+        u64 = None
+        for pol in (P.FP64FP64, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED):
+            wl = taylor_green_3d(shape, pol, nu=nu)
+            vs = wl.stepper.velocity_set
+            _, _, f0, f1 = time_run(torch, wl, steps, 0)
+            u = decoded_velocity(torch, f0, pol, vs)
+            if u64 is None:
+                u64 = u
+                row["ke_rms_u"] = float(u.pow(2).sum(0).mean().sqrt())
+            else:
+                row[pol.name] = float((u - u64).norm() / u64.norm())
+            del wl, f0, f1, u
+            torch.cuda.empty_cache()
+        res[f"nu={nu}"] = row
+        log(f"bench: precision sweep nu={nu}: " + ", ".join(f"{k}={v:.3e}" for k, v in row.items()))
+        del u64
+        torch.cuda.empty_cache()
+    return dict(workload=f"periodic D3Q19 BGK Taylor-Green vortex {shape[0]}x{shape[1]}x{shape[2]}, u0=0.04, "
+                         f"{steps} steps, viscosities {list(nus)}; shifted 16-bit formats against FP64", rows=res)
+
+
 def headline_precision_leg(xa, torch, n=256, steps=1000):
     """Velocity error of every storage format on the headline cavity after ``steps`` steps,
     against the FP64/FP64 run of the same workload."""
@@ -651,6 +681,7 @@ def main():
             "mesh_masker": lambda: mesh_leg(xa, torch, subdivisions=4 if args.quick else 6),
             "adjoint": lambda: adjoint_leg(xa, torch, quick=args.quick),
             "periodic": lambda: periodic_leg(xa, torch, n, args.steps, args.warmup),
+            "precision_viscosity_sweep": lambda: precision_viscosity_sweep(xa, torch),
         }
         names = args.only.split(",")
         unknown = [k for k in names if k not in legs]
@@ -679,6 +710,7 @@ def main():
         pshape = (128, 64, 64) if args.quick else (512, 256, 256)
         extras["precision"] = precision_leg(xa, torch, pshape, 200 if args.quick else 2000)
         extras["precision_headline"] = headline_precision_leg(xa, torch, 128 if args.quick else n, 1000)
+        extras["precision_viscosity_sweep"] = precision_viscosity_sweep(xa, torch, pshape, 200 if args.quick else 2000)
         extras["adjoint"] = adjoint_leg(xa, torch, quick=args.quick)
         extras["kbc_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick)
         extras["grads_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick, sphere_bc="grads")
