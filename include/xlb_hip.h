@@ -13,6 +13,10 @@
  * (card, nx, ny, 1).  `N = nx*ny*nz` is the number of cells.  Population l of cell
  * (x,y,z) lives at element  l*N + (x*ny + y)*nz + z.
  *
+ * Grid limits: 1 <= nx <= 65535 and N <= 2^32 - 256; entry points that build cell lists
+ * (xlbh_compact_cells and the adjoint's boundary pass) need N < 2^31.  Planes above 4 GiB are
+ * addressed with 64-bit offsets.
+ *
  * Lattice direction order, weights and opposite indices follow
  * xlb/velocity_set/d2q9.py:16-24, d3q19.py:17-30, d3q27.py:17-32.
  */

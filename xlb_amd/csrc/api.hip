@@ -15,7 +15,8 @@ int check_grid(int lattice, int nx, int ny, int nz) {
   XLBH_REQUIRE(nx > 0 && ny > 0 && nz > 0, "grid extents must be positive");
   XLBH_REQUIRE(lattice != XLBH_D2Q9 || nz == 1, "2-D lattices use fields of shape (q, nx, ny, 1)");
   XLBH_REQUIRE(lattice == XLBH_D2Q9 || nz > 1, "3-D lattices need nz > 1");
-  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 32), "more than 2^32 - 1 cells per field");
+  // 1-D launches round the cell count up to whole 256-thread blocks in 32-bit arithmetic.
+  XLBH_REQUIRE((int64_t)nx * ny * nz <= ((int64_t)1 << 32) - 256, "more than 2^32 - 256 cells per field");
   XLBH_REQUIRE(nx <= 65535, "nx must be <= 65535");
   return 0;
 }
