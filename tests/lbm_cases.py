@@ -258,6 +258,12 @@ CASES = {
     "cavity_grads": ("D3Q19", (16, 16, 16), "BGK", 1.6, "random"),
     "sphere3d_grads": ("D3Q27", (40, 24, 24), "KBC", 1.9, "rest"),
     "sphere3d_grads_bgk": ("D3Q19", (40, 24, 24), "BGK", 1.7, "rest"),
+    # smallest grids: every pull wraps onto the cell itself or its only neighbour
+    "tiny_periodic3d": ("D3Q19", (1, 1, 2), "BGK", 1.5, "random"),
+    "tiny_periodic2d": ("D2Q9", (1, 1), "BGK", 1.3, "random"),
+    "tiny_periodic_kbc": ("D3Q27", (2, 1, 3), "KBC", 1.8, "random"),
+    "tiny_cavity2d": ("D2Q9", (3, 3), "BGK", 1.5, "rest"),
+    "tiny_cavity3d": ("D3Q19", (3, 3, 3), "BGK", 1.0, "rest"),
 }
 
 
@@ -272,7 +278,8 @@ LARGE_CASES = {
 }
 BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "c3_tgv": "periodic", "cavity_smagorinsky": "cavity_halfway",
           "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d",
-          "sphere3d_grads_bgk": "sphere3d_grads"}
+          "sphere3d_grads_bgk": "sphere3d_grads", "tiny_periodic3d": "periodic", "tiny_periodic2d": "periodic",
+          "tiny_periodic_kbc": "periodic", "tiny_cavity2d": "cavity2d", "tiny_cavity3d": "cavity_fullway"}
 
 ORACLE_COLLISION = {"BGK": 0, "KBC": 1, "SmagorinskyLESBGK": 2}
 

@@ -56,6 +56,25 @@ def test_error_channel_and_validation_without_gpu():
     # unknown lattice
     rc = L.xlbh_stream(7, _lib.F32, ctypes.c_void_p(8), ctypes.c_void_p(16), 4, 4, 4, None)
     assert rc != 0 and b"lattice" in L.xlbh_last_error()
+    # grid limits: empty extents, nx above the launch grid's z range, more than 2^32 - 256 cells
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    for (nx, ny, nz), msg in [((0, 8, 8), b"positive"), ((8, 8, -1), b"positive"), ((65536, 2, 2), b"65535"),
+                              ((65535, 65536, 2), b"2^32 - 256")]:
+        rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
+                         _lib.FORCE_NONE, None, None, nx, ny, nz, 1.0, 0.0, 0, None)
+        assert rc != 0 and msg in L.xlbh_last_error(), (nx, ny, nz, L.xlbh_last_error())
+    # the largest 2-D grid that fits passes validation: 65535 x 65535 cells = 2^32 - 2^17 + 1
+    rc = L.xlbh_run(_lib.D2Q9, _lib.BGK, _lib.F16, _lib.COMPUTE_F32, a, b, None, None, None, None,
+                    _lib.FORCE_NONE, None, None, 65535, 65535, 1, 1.0, 0.0, 0, 0, None)
+    assert rc == 0, L.xlbh_last_error()
+    # negative step counts
+    rc = L.xlbh_run(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
+                    _lib.FORCE_NONE, None, None, 4, 4, 4, 1.0, 0.0, 0, -1, None)
+    assert rc != 0 and b"n_steps" in L.xlbh_last_error()
+    # a boundary mask without its packed bits and table
+    rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, ctypes.c_void_p(0x3000), None, None,
+                     None, _lib.FORCE_NONE, None, None, 4, 4, 4, 1.0, 0.0, 0, None)
+    assert rc != 0 and b"missing bits" in L.xlbh_last_error()
     # workspace sizes
     assert L.xlbh_reduce_workspace_bytes(256) == 4 * 8
     assert L.xlbh_compact_workspace_bytes(512) == 2 * 2 * 4

@@ -233,3 +233,33 @@ def test_grads_bc_on_a_mesh_is_rejected():
     st = xa.operator.stepper.IncompressibleNavierStokesStepper(grid, [bc])
     with pytest.raises(Exception, match="MeshDistanceBoundaryMasker"):
         st.prepare_fields()
+
+
+def test_zero_steps_leave_the_fields_untouched():
+    """run(..., 0) launches nothing and returns (f_0, f_1) in their original roles."""
+    pair = Pair("cavity_fullway", P.FP32FP32)
+    pair.f1.fill_(7.0)
+    before0, before1 = to_np(pair.f0).copy(), to_np(pair.f1).copy()
+    f0, f1 = pair.stepper.run(pair.f0, pair.f1, pair.bc_mask, pair.missing, pair.omega, 0)
+    assert f0 is pair.f0 and f1 is pair.f1
+    np.testing.assert_array_equal(to_np(f0), before0)
+    np.testing.assert_array_equal(to_np(f1), before1)
+
+
+def test_boundary_with_no_indices_changes_nothing():
+    """A boundary condition given an empty index set marks no cell: the step equals the periodic
+    step of the same populations bit for bit."""
+    BC = xa.operator.boundary_condition
+    a = Pair("periodic", P.FP32FP32)
+    shape = a.grid.shape
+    empty = [[], [], []]
+    st = xa.operator.stepper.IncompressibleNavierStokesStepper(a.grid, [BC.HalfwayBounceBackBC(indices=empty)])
+    f0, f1, bc_mask, missing = st.prepare_fields()
+    assert int(bc_mask.sum().item()) == 0 and not bool(missing.any().item())
+    f0.copy_(a.f0)
+    for t in range(5):
+        f0, f1 = st(f0, f1, bc_mask, missing, a.omega, t)
+        f0, f1 = f1, f0
+    a.step_product(5)
+    assert tuple(f0.shape) == (19,) + tuple(shape)
+    np.testing.assert_array_equal(to_np(f0), to_np(a.f0))
