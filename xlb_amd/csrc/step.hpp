@@ -177,17 +177,18 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
   const Planes<R, O> fout{static_cast<const char*>(p.f_out), plane_bytes};
   const PullOffsets<O> po(x, y, z, nx, ny, nz);
 
-  // All pulls are issued before the boundary id is examined, so the id load does not add a
-  // round trip in front of them (every source address is in bounds for every cell).
+  // The boundary id is loaded first and the pulls right after it, with no wait in between (every
+  // source address is in bounds for every cell).  The empty asm statement keeps the compiler from
+  // sinking the pulls below the solid-cell exit, which would put a round trip in front of them.
+  // Loads complete in issue order, so a boundary cell's table and missing-bit loads, which need
+  // only the id, are issued while its pulls are still in flight.
+  int id = 0;
+  if (p.bc_mask != nullptr) id = __builtin_nontemporal_load(p.bc_mask + cell);
   C f[Q];
 #pragma unroll
   for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fin.at(l, po.template src<LAT>(l)), T.w[l]);
-
-  int id = 0;
-  if (p.bc_mask != nullptr) {
-    id = p.bc_mask[cell];
-    if (id == 255) return;  // solid: no update (nse_stepper.py:356-358)
-  }
+  asm volatile("" ::: "memory");
+  if (id == 255) return;  // solid: no update (nse_stepper.py:356-358)
   cell_update<LAT, COLL, FMT, C, O>(f, id, cell, x, y, z, p, fin);
 #pragma unroll
   for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout.at_w(l, cell), f[l], T.w[l]);
