@@ -87,13 +87,18 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
   int kind = XLBH_BC_NONE;
   uint32_t mm = 0;
   if (id != 0) {
+    // The whole descriptor and the missing bits are loaded together, ahead of the kind
+    // branches (the empty asm statement keeps the compiler from sinking the EquilibriumBC
+    // constants into their branch, which would cost those cells a second round trip).
     const xlbh_bc_desc* d = p.table + id;
     kind = d->kind;
+    const double drho = d->rho, du0 = d->u[0], du1 = d->u[1], du2 = d->u[2];
     mm = p.missing[cell];
+    asm volatile("" ::: "memory");
     // ---- streaming-step boundary conditions (apply_bc with is_post_streaming) ----
     if (kind == XLBH_BC_EQUILIBRIUM) {
-      const C rb = (C)d->rho;
-      const C ub[3] = {(C)d->u[0], (C)d->u[1], (C)d->u[2]};
+      const C rb = (C)drho;
+      const C ub[3] = {(C)du0, (C)du1, (C)du2};
       const C us = usqr15<LAT>(ub);
 #pragma unroll
       for (int l = 0; l < Q; ++l) f[l] = feq_l<LAT>(l, rb, ub, us);
