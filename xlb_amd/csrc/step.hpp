@@ -200,13 +200,21 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
 }
 
 // Occupancy.  Measured on the 256^3 D3Q19 sweep: 4-byte populations stream best at 4 waves per
-// SIMD (more waves add concurrent streams without adding bandwidth); 2-byte populations want as
-// many waves as the registers allow.  The 4- and 8-byte kernels are launched with an LDS
+// SIMD (more waves add concurrent streams without adding bandwidth: 6 waves lost 3 %); 2-byte
+// populations want more waves than that.  The 4- and 8-byte kernels are launched with an LDS
 // reservation (kStepLdsCap, otherwise unused) that admits 4 blocks = 4 waves per SIMD per CU.
+// The D3Q19 2-byte kernels with FP32 arithmetic are compiled for 6 waves per SIMD (80 VGPRs, no
+// spills; the default allocation took 82 VGPRs = 5 waves): +2 % on the cavity, whose boundary
+// lookups the extra waves hide, and -1 % on a periodic box.  D3Q27 would spill at 80 VGPRs.
 constexpr unsigned kStepLdsCap = 36 * 1024;
 
 template <int LAT, int COLL, int FMT, class C, bool WIDE>
 __global__ __launch_bounds__(256) void k_step(StepParams p) {
+  step_body<LAT, COLL, FMT, C, WIDE>(p);
+}
+
+template <int LAT, int COLL, int FMT, class C, bool WIDE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_step_w6(StepParams p) {
   step_body<LAT, COLL, FMT, C, WIDE>(p);
 }
 
@@ -247,8 +255,11 @@ int launch_step_t(const StepCall& c) {
   p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
   const bool wide = (uint64_t)p.N * sizeof(typename Fmt<FMT>::raw) > 0xffffffffull;
   const unsigned lds = sizeof(typename Fmt<FMT>::raw) >= 4 ? kStepLdsCap : 0u;
+  constexpr bool w6 = LAT == XLBH_D3Q19 && sizeof(typename Fmt<FMT>::raw) == 2 && std::is_same<C, float>::value;
   if (wide)
     hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+  else if constexpr (w6)
+    hipLaunchKernelGGL((k_step_w6<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
   else
     hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
   XLBH_LAUNCH_CHECK();
