@@ -420,24 +420,35 @@ struct AdjParams {
   uint32_t N;
 };
 
-// The streaming-step view of one cell: post-BC populations h, the adjoint of the output with the
-// slots overwritten after collision removed, and the BC kind.  Shared by every adjoint kernel.
-// `lamf`, when given, is the cell's dL/df_out (otherwise it is read from lam_out).
+// The populations the forward step pulled into cell (x, y, z), f_ps (periodic wrap).
 template <int LAT, int FMT, class C>
-__device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, int x, int y, int z, int kind,
-                                            uint32_t mm, const xlbh_bc_desc* d, C* fps, C* h, C* lam,
-                                            const C* lamf = nullptr) {
+__device__ __forceinline__ void adj_pull(const AdjParams& p, int x, int y, int z, C* fps) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   using R = typename Fmt<FMT>::raw;
   const R* fin = static_cast<const R*>(p.f_in);
-  const C* lo = static_cast<const C*>(p.lam_out);
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
 #pragma unroll
   for (int l = 0; l < Q; ++l) {
     const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
     fps[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
-    lam[l] = lamf ? lamf[l] : lo[(size_t)l * p.N + cell];
+  }
+}
+
+// The streaming-step view of one cell: post-BC populations h, the adjoint of the output with the
+// slots overwritten after collision removed, and the BC kind, from the pulled populations fps
+// (adj_pull) and the cell's dL/df_out lamf.  Shared by every adjoint kernel.
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ void adj_prepare_pulled(const AdjParams& p, uint32_t cell, int kind, uint32_t mm,
+                                                   const xlbh_bc_desc* d, const C* fps, C* h, C* lam,
+                                                   const C* lamf) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  using R = typename Fmt<FMT>::raw;
+  const R* fin = static_cast<const R*>(p.f_in);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    lam[l] = lamf[l];
     h[l] = fps[l];
   }
   if (kind == XLBH_BC_EQUILIBRIUM) {
@@ -465,6 +476,19 @@ __device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, i
     for (int l = 0; l < Q; ++l)
       if (bit(mm, l)) lam[T.opp[l]] = C(0);  // collision output of slot opp(l) is replaced by the stash
   }
+}
+
+// adj_prepare_pulled with the pulls and dL/df_out loaded here.
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ void adj_prepare(const AdjParams& p, uint32_t cell, int x, int y, int z, int kind,
+                                            uint32_t mm, const xlbh_bc_desc* d, C* fps, C* h, C* lam) {
+  constexpr int Q = make_tables(LAT).q;
+  const C* lo = static_cast<const C*>(p.lam_out);
+  C lamf[Q];
+  adj_pull<LAT, FMT, C>(p, x, y, z, fps);
+#pragma unroll
+  for (int l = 0; l < Q; ++l) lamf[l] = lo[(size_t)l * p.N + cell];
+  adj_prepare_pulled<LAT, FMT, C>(p, cell, kind, mm, d, fps, h, lam, lamf);
 }
 
 template <int LAT, class C>
@@ -710,7 +734,18 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
   if (!sweep_cell(p.sw, x, y, z)) return;
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
   const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
-  const int id = p.bc_mask ? p.bc_mask[cell] : 0;
+  // Every load that needs only the cell's position goes out first, in one batch: the boundary id,
+  // the neighbour-delta flag, dL/df_out and the pulled populations.  The empty asm statement keeps
+  // the compiler from sinking them below the id-dependent branches (which would cost one round
+  // trip per branch); lam_in, lam_out, f_in and the delta buffers are distinct allocations.
+  const int id = p.bc_mask ? __builtin_nontemporal_load(p.bc_mask + cell) : 0;
+  const int nbf = p.nb_flag[cell];
+  const C* lo = static_cast<const C*>(p.lam_out);
+  C lamf[Q], fps[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) lamf[l] = lo[(size_t)l * p.N + cell];
+  adj_pull<LAT, FMT, C>(p, x, y, z, fps);
+  asm volatile("" ::: "memory");
   C* li = static_cast<C*>(p.lam_in);
   if (id == 255) {
     // solid cells are not updated by the forward step: nothing flows back through their pulls
@@ -730,16 +765,12 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
     mm = p.missing[cell];
   }
   // full dL/df_out at this cell: the pushes of the previous reverse step plus its deltas
-  const C* lo = static_cast<const C*>(p.lam_out);
   C* dob = static_cast<C*>(p.d_own);
-  C lamf[Q];
-#pragma unroll
-  for (int l = 0; l < Q; ++l) lamf[l] = lo[(size_t)l * p.N + cell];
   if (own_kind(kind)) {
 #pragma unroll
     for (int l = 0; l < Q; ++l) lamf[l] += dob[(size_t)l * p.N + cell];
   }
-  if (p.nb_flag[cell]) {
+  if (nbf) {
     C* dni = static_cast<C*>(p.d_nb_in);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -747,8 +778,8 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
       dni[(size_t)l * p.N + cell] = C(0);
     }
   }
-  C fps[Q], h[Q], lam[Q];
-  adj_prepare<LAT, FMT, C>(p, cell, x, y, z, kind, mm, d, fps, h, lam, lamf);
+  C h[Q], lam[Q];
+  adj_prepare_pulled<LAT, FMT, C>(p, cell, kind, mm, d, fps, h, lam, lamf);
   C G[Q], dn[Q];
 #pragma unroll
   for (int k = 0; k < Q; ++k) dn[k] = C(0);
