@@ -58,6 +58,7 @@ def lib():
         L.orc_momentum_transfer.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, i, i, vp]
         L.orc_lattice.argtypes = [i, vp, vp, vp, vp]
         L.orc_set_threads.argtypes = [i]
+        L.orc_plant.argtypes = [i]
         L.orc_f32_to_f16.argtypes = [vp, vp, i]
         L.orc_f32_to_bf16.argtypes = [vp, vp, i]
         _LIB = L
@@ -82,6 +83,30 @@ def _bcs(bcs):
 
 def _dims(f):
     return int(f.shape[1]), int(f.shape[2]), int(f.shape[3])
+
+
+# Planted slips (orc_plant): each pinning test switches one on to show that its check fails.
+PLANT_KBC_GAMMA_SIGN = 1
+PLANT_KBC_SHEAR_INDEX = 2
+PLANT_MOMENTUM_OPP_INDEX = 3
+PLANT_MOMENTUM_NO_STREAM_TERM = 4
+PLANT_EXTRAPOLATION_WEIGHTS = 5
+PLANT_EXTRAPOLATION_SOURCE = 6
+
+
+class planted:
+    """``with planted(PLANT_...):`` runs the oracle with that transcription slip switched on."""
+
+    def __init__(self, which):
+        self.which = int(which)
+
+    def __enter__(self):
+        self.old = lib().orc_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant(self.old)
+        return False
 
 
 def set_threads(n):

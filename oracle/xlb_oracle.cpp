@@ -14,20 +14,34 @@
 //
 // Every function is pinned by tests/test_oracle_cpu.py, tests/test_mesh_cpu.py and the golden
 // fixtures (known-answer tests of the reference's own test suite, lattice identities,
-// conservation, closed-form flows and special cases).  One aspect is parity unpinned: bit-level
+// conservation, closed-form flows and special cases).  Where a check was written to catch a
+// particular transcription slip, the slip can be switched on here (orc_plant, off by default) so
+// the test shows that its check fails on it.  One aspect is parity unpinned: bit-level
 // agreement of the triangle / box test (orc_tri_box) with the reference's Warp formulation of it,
 // whose matrix-from-vectors convention cannot be run here.  The test itself is pinned against an
 // independent float64 separating-axis test and a closed-form box voxelisation.
 //
 // Build: see oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include <omp.h>
 
 namespace {
+
+// Planted transcription slips, off (0) except while a pinning test shows that its check fails on
+// one of them (orc_plant):
+//   1  KBC: sign of the (2 - 1/beta) term of gamma flipped
+//   2  KBC: two shear-basis assignments exchanged (s[12] <-> s[15] in D3Q27, s[8] <-> s[4] in D2Q9)
+//   3  momentum exchange: f_pc[l] taken for f_pc[opp l]
+//   4  momentum exchange: the post-streaming term f_ps[l] dropped
+//   5  extrapolation outflow: weights cs and 1 - cs exchanged
+//   6  extrapolation outflow: source x - c_l instead of x - c_l - n
+int g_plant = 0;
 
 // ------------------------------------------------------------------------------------------
 // Lattices, written out from the reference's constructors (d2q9.py:16-24; d3q19.py:17-30 =
@@ -327,6 +341,7 @@ void kbc_shear(const Lattice& L, const C* fneq, C* s) {
     s[24] = pi[1] / four;
     s[21] = -pi[1] / four;
     s[15] = -pi[1] / four;
+    if (g_plant == 2) std::swap(s[12], s[15]);
     s[10] = pi[2] / four;
     s[20] = pi[2] / four;
     s[19] = -pi[2] / four;
@@ -345,6 +360,7 @@ void kbc_shear(const Lattice& L, const C* fneq, C* s) {
     s[4] = -pi[1];
     s[5] = -pi[1];
     s[7] = pi[1];
+    if (g_plant == 2) std::swap(s[8], s[4]);
   }
 }
 
@@ -389,7 +405,8 @@ void collision(const Lattice& L, int coll, const C* f, const C* feq, C rho, cons
     sp2 += temp * dh[l];
   }
   const C eps = (C)1e-32;
-  const C gamma = inv_beta - ((C)2.0 - inv_beta) * sp1 / (eps + sp2);
+  const C gamma = g_plant == 1 ? inv_beta + ((C)2.0 - inv_beta) * sp1 / (eps + sp2)
+                               : inv_beta - ((C)2.0 - inv_beta) * sp1 / (eps + sp2);
   for (int l = 0; l < L.q; ++l) out[l] = f[l] - beta * ((C)2.0 * ds[l] + gamma * dh[l]);
 }
 
@@ -541,11 +558,11 @@ void extrapolation_aux(const Ctx<FMT, C>& X, int x, int y, int z, const uint8_t*
     if (mm[l] == 1) {
       int idx[3] = {x, y, z}, p[3] = {x, y, z};
       for (int d = 0; d < L.d; ++d) {
-        p[d] = idx[d] - (L.c[d][l] + nv[d]);
+        p[d] = idx[d] - (L.c[d][l] + (g_plant == 6 ? 0 : nv[d]));
         p[d] = ((p[d] % shape[d]) + shape[d]) % shape[d];
       }
       const C faux = X.rd(X.f0, l, X.cell(p[0], p[1], p[2]));
-      out[L.opp[l]] = ((C)1.0 - cs) * f_pre[l] + cs * faux;
+      out[L.opp[l]] = g_plant == 5 ? cs * f_pre[l] + ((C)1.0 - cs) * faux : ((C)1.0 - cs) * f_pre[l] + cs * faux;
     }
   }
 }
@@ -671,6 +688,13 @@ int orc_lattice(int lat, int* c /*3*27*/, double* w, int* opp, double* qi /*27*6
     for (int t = 0; t < 6; ++t) qi[i * 6 + t] = L.qi[i][t];
   }
   return L.q;
+}
+
+// Test hook: switch on planted slip `which` (0 = none); returns the previous setting.
+int orc_plant(int which) {
+  const int old = g_plant;
+  g_plant = which;
+  return old;
 }
 
 int orc_set_threads(int n) {
@@ -1033,7 +1057,7 @@ int orc_momentum_transfer(int lat, int store, int compute, const OrcBc* bc, cons
             C m = 0;
             for (int l = 0; l < L.q; ++l)
               if (mm[l] == 1) {
-                const C phi = fpc[L.opp[l]] + tmp[l];
+                const C phi = g_plant == 3 ? fpc[l] + tmp[l] : (g_plant == 4 ? fpc[L.opp[l]] : fpc[L.opp[l]] + tmp[l]);
                 if (L.c[d][L.opp[l]] == 1) m += phi;
                 else if (L.c[d][L.opp[l]] == -1) m -= phi;
               }

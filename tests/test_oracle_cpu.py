@@ -374,6 +374,71 @@ def test_kbc_relaxes_shear_like_bgk(lat):
     np.testing.assert_allclose(orc.collide(lat, 1, orc.F64, 1, feq, feq, rho, u, 1.6), feq, atol=1e-15)
 
 
+def _natural_shear(c, fneq, rho):
+    """The shear part of fneq in KBC's natural-moment decomposition, built from its definition
+    rather than the reference's index list (kbc.py:192-250): with Pi the second moment of fneq and
+    D = Pi - tr(Pi)/d its deviatoric part, a main direction along axis a carries rho D_aa / 2, a
+    diagonal in the (a, b) plane carries rho c_a c_b Pi_ab / 4, every other direction nothing.  The
+    second moment of the result is rho D (both lattices: the reference's D3Q27 rho s and D2Q9
+    rho s / 4 land on this same vector)."""
+    d = c.shape[0]
+    pi = np.einsum("al,bl,l...->ab...", c, c, fneq)
+    tr = np.trace(pi)
+    ds = np.zeros_like(fneq)
+    for l in range(c.shape[1]):
+        axes = np.flatnonzero(c[:, l])
+        if len(axes) == 1:
+            a = axes[0]
+            ds[l] = rho * (pi[a, a] - tr / d) / 2
+        elif len(axes) == 2:
+            a, b = axes
+            ds[l] = rho * c[a, l] * c[b, l] * pi[a, b] / 4
+    return ds
+
+
+def _kbc_residuals(c, omega, f, feq, rho, out):
+    """KBC writes f_out = f - beta (2 ds + gamma dh) with dh = fneq - ds, beta = omega / 2, and
+    chooses gamma so that the post-collision state is stationary for the entropy along dh:
+    <dh, f_out - feq> = 0 in the scalar product weighted by 1/feq (kbc.py:252-296).  Returns
+    (distance of f_out - feq - (1 - 2 beta) ds from the line through dh, the stationarity
+    residual), both relative.  Together they fix gamma."""
+    fneq = f - feq
+    ds = _natural_shear(c, fneq, rho)
+    dh = fneq - ds
+    r = out - feq - (1 - omega) * ds  # = (1 - beta gamma) dh
+    k = (r * dh).sum(0) / (dh * dh).sum(0)
+    line = np.abs(r - k * dh).max() / np.abs(r).max()
+    stationary = np.abs((dh * (out - feq) / feq).sum(0)).max() / np.abs((dh * dh / feq).sum(0)).max()
+    return line, stationary
+
+
+@pytest.mark.parametrize("lat", [0, 2])
+def test_kbc_gamma_is_the_entropic_stationary_point(lat):
+    """Pins KBC's entropic stabiliser gamma (kbc.py:252-296) for any rho, at omega in {0.8, 1.3,
+    1.9}: f_out - feq - (1 - omega) ds lies on the line through dh and is orthogonal to dh in the
+    1/feq-weighted product, with ds from the natural-moment definition above.  A flipped sign of the
+    (2 - 1/beta) term or two exchanged shear assignments, planted in the oracle, fail it."""
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 5)
+    rng = np.random.default_rng(16)
+    c, w = ref_lattice(lat)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, 1 + 0.05 * rng.standard_normal(shp), 0.05 * rng.standard_normal((d,) + shp), f)
+    f += 2e-3 * rng.standard_normal(f.shape) * w.reshape(-1, 1, 1, 1)
+    rho = f.sum(0)
+    u = np.tensordot(c, f, axes=(1, 0)) / rho
+    feq = np.zeros_like(f)
+    orc.equilibrium(lat, orc.F64, 1, rho, u, feq)
+    for omega in (0.8, 1.3, 1.9):
+        line, stationary = _kbc_residuals(c, omega, f, feq, rho, orc.collide(lat, 1, orc.F64, 1, f, feq, rho, u, omega))
+        assert line < 1e-10 and stationary < 1e-12, (omega, line, stationary)
+        for slip in (orc.PLANT_KBC_GAMMA_SIGN, orc.PLANT_KBC_SHEAR_INDEX):
+            with orc.planted(slip):
+                bad = orc.collide(lat, 1, orc.F64, 1, f, feq, rho, u, omega)
+            line, stationary = _kbc_residuals(c, omega, f, feq, rho, bad)
+            assert max(line, stationary) > 1e-3, (slip, omega, line, stationary)
+
+
 @pytest.mark.parametrize("lat", [0, 1, 2])
 def test_smagorinsky_pinned(lat):
     """SmagorinskyLESBGK (smagorinsky_les_bgk.py:73-89) pinned by what its definition fixes:
@@ -431,6 +496,53 @@ def test_momentum_exchange_symmetric_body_has_no_lift():
     F = orc.momentum_transfer(lat, orc.F64, 1, dict(id=1, kind=orc.HALFWAY_BB), out, bc_mask, miss)
     assert F[0] > 0  # drag along the flow
     assert abs(F[1]) < 1e-12 * abs(F[0]) + 1e-15 and abs(F[2]) < 1e-12 * abs(F[0]) + 1e-15
+
+
+def _momentum_balance(lat, coll):
+    """A periodic FP64 box with a halfway-bounce-back body (every voxel an interior index, so the
+    masker pads it) in a perturbed flow, 3 steps in.  Returns (change of the fluid's momentum over
+    one more step, momentum-exchange force of the state before that step)."""
+    d, q = LATS[lat]
+    shp = grid_shape(lat, 14)
+    g = np.indices(shp).astype(float)
+    ctr = (6.3, 7.1, 6.6)
+    inside = sum((g[a] - ctr[a]) ** 2 for a in range(d)) < 3.2**2
+    idx = np.array(np.where(inside), dtype=np.int32)
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.ones(idx.shape[1], np.uint8), 1, bc_mask, miss)
+    rng = np.random.default_rng(3)
+    c, w = ref_lattice(lat)
+    u = np.array([0.03, -0.01, 0.02][:d]).reshape((d, 1, 1, 1)) + 0.01 * rng.standard_normal((d,) + shp)
+    f = np.zeros((q,) + shp)
+    orc.equilibrium(lat, orc.F64, 1, 1 + 0.01 * rng.standard_normal(shp), u, f)
+    f += 1e-3 * rng.standard_normal(f.shape) * w.reshape(-1, 1, 1, 1)
+    bcs = [dict(id=1, kind=orc.HALFWAY_BB)]
+    f = _run(lat, coll, f, 1.6, 3, bc_mask, miss, bcs)
+    # the fluid: id 0 and the body's neighbours (boundary cells whose rest direction is not missing)
+    fluid = (bc_mask[0] == 0) | ((bc_mask[0] == 1) & (miss[0] == 0))
+    force = orc.momentum_transfer(lat, orc.F64, 1, bcs[0], f, bc_mask, miss)[:d]
+    p0 = np.tensordot(c, f, axes=(1, 0))[:, fluid].sum(1)
+    p1 = np.tensordot(c, _run(lat, coll, f, 1.6, 1, bc_mask, miss, bcs), axes=(1, 0))[:, fluid].sum(1)
+    return p1 - p0, force
+
+
+@pytest.mark.parametrize("lat,coll", [(1, 0), (2, 1), (0, 0), (0, 1)])
+def test_momentum_exchange_balances_fluid_momentum(lat, coll):
+    """Pins the momentum-exchange magnitude (momentum_transfer.py:145-160): the force is the
+    momentum the body takes from the fluid, so over one step of a periodic box without body force
+    the fluid's momentum changes by exactly -F(state before the step).  Streaming between fluid
+    cells and the momentum-conserving collision change nothing; each population f_pc[opp l] that
+    would enter the body is bounced back into l, taking 2 c_{opp l} f_pc[opp l], which is what
+    f_pc[opp l] + f_ps[l] sums for halfway bounce-back.  Taking f_pc[l] for f_pc[opp l], or dropping
+    the f_ps term, planted in the oracle, fails it."""
+    dp, force = _momentum_balance(lat, coll)
+    assert np.abs(force).max() > 1e-3
+    np.testing.assert_allclose(dp, -force, rtol=0, atol=1e-12 * np.abs(force).max())
+    for slip in (orc.PLANT_MOMENTUM_OPP_INDEX, orc.PLANT_MOMENTUM_NO_STREAM_TERM):
+        with orc.planted(slip):
+            dp, force = _momentum_balance(lat, coll)
+        assert np.abs(dp + force).max() > 0.1 * np.abs(force).max(), slip
 
 
 # ------------------------------------------------------------------ boundary-condition closed forms
@@ -607,6 +719,56 @@ def test_extrapolation_outflow_keeps_uniform_flow(lat):
         expect[opp[l]][m] = f[l][m]
     out = _run(lat, 0, expect, 1.3, 12, bc_mask, miss, bcs)
     np.testing.assert_allclose(out, expect, atol=1e-14)
+
+
+def _extrapolation_error(lat, axis, side, seed=21):
+    """Largest distance, over the outlet's missing directions, between the oracle's stashed value
+    and its definition as an upwind interpolation (bc_extrapolation_outflow.py:163-195): the
+    post-streaming value of a missing direction l at the outlet (bounce-back: the own-cell f[opp l])
+    and at the cell one step upstream along -n (numpy.roll of f[l] by c_l, then by n) are the
+    endpoints of a segment of length 1, and the stash in slot opp l is the segment's value at
+    distance c_s from the outlet, c_s = sqrt(sum_l w_l c_lx^2) the lattice sound speed."""
+    d, q = LATS[lat]
+    c, w = ref_lattice(lat)
+    opp = orc.lattice(lat)["opp"]
+    cs = np.sqrt((w * c[0] ** 2).sum())
+    n = 7
+    shp = grid_shape(lat, n)
+    g = np.indices(shp)
+    face = g[axis] == (n - 1 if side > 0 else 0)
+    for b in range(d):
+        if b != axis:
+            face &= (g[b] > 0) & (g[b] < n - 1)
+    idx = np.array(np.where(face), dtype=np.int32)
+    bc_mask = np.zeros((1,) + shp, np.uint8)
+    miss = np.zeros((q,) + shp, np.uint8)
+    orc.mask_indices(lat, idx, np.zeros(idx.shape[1], np.uint8), 1, bc_mask, miss)
+    f = _random_pops(lat, shp, seed)
+    out = _run(lat, 0, f, 1.3, 1, bc_mask, miss, [dict(id=1, kind=orc.EXTRAPOLATION_OUTFLOW)])
+    normal = tuple(side if a == axis else 0 for a in range(d))
+    err, checked = 0.0, 0
+    for l in range(q):
+        m = miss[l].astype(bool) & face
+        post = np.roll(f[l], shift=tuple(int(v) for v in c[:, l]), axis=tuple(range(d)))
+        upstream = np.roll(post, shift=normal, axis=tuple(range(d)))
+        for x in zip(*np.where(m)):
+            expect = np.interp(-cs, [-1.0, 0.0], [upstream[x], f[opp[l]][x]])
+            err = max(err, abs(out[opp[l]][x] - expect))
+            checked += 1
+    assert checked == face.sum() * (3 if d == 2 else (5 if q == 19 else 9))
+    return err
+
+
+@pytest.mark.parametrize("lat", [0, 1, 2])
+@pytest.mark.parametrize("axis,side", [(0, 1), (0, -1), (1, 1)])
+def test_extrapolation_outflow_is_upwind_interpolation(lat, axis, side):
+    """Pins ExtrapolationOutflowBC's c_s weight and its x - c_l - n source on random populations
+    (outlets on x = n-1, x = 0 and y = n-1, without edges, so n is the face normal).  Exchanging
+    the weights c_s and 1 - c_s, or dropping n from the source, planted in the oracle, fails it."""
+    assert _extrapolation_error(lat, axis, side) < 1e-15
+    for slip in (orc.PLANT_EXTRAPOLATION_WEIGHTS, orc.PLANT_EXTRAPOLATION_SOURCE):
+        with orc.planted(slip):
+            assert _extrapolation_error(lat, axis, side) > 1e-4, slip
 
 
 def test_do_nothing_skips_streaming():
