@@ -27,38 +27,29 @@ class Grid:
         self.compute_backend = compute_backend
 
     def bounding_box_indices(self, remove_edges=False):
-        """Index lists of the 4 (2-D) or 6 (3-D) faces; grid.py:34-86 (same keys, order and lists)."""
-        origin = np.array([0, 0, 0])
-        bounds = np.array(self.shape)
-        if remove_edges:
-            origin += 1
-            bounds -= 1
-        slice_x = slice(origin[0], bounds[0])
-        slice_y = slice(origin[1], bounds[1])
-        dim = len(bounds)
-        grid = np.indices(self.shape)
-        if dim == 2:
-            nx, ny = self.shape
-            box = {
-                "bottom": grid[:, slice_x, 0],
-                "top": grid[:, slice_x, ny - 1],
-                "left": grid[:, 0, slice_y],
-                "right": grid[:, nx - 1, slice_y],
-            }
-        elif dim == 3:
-            nx, ny, nz = self.shape
-            slice_z = slice(origin[2], bounds[2])
-            box = {
-                "bottom": grid[:, slice_x, slice_y, 0].reshape(3, -1),
-                "top": grid[:, slice_x, slice_y, nz - 1].reshape(3, -1),
-                "left": grid[:, 0, slice_y, slice_z].reshape(3, -1),
-                "right": grid[:, nx - 1, slice_y, slice_z].reshape(3, -1),
-                "front": grid[:, slice_x, 0, slice_z].reshape(3, -1),
-                "back": grid[:, slice_x, ny - 1, slice_z].reshape(3, -1),
-            }
-        else:
-            raise ValueError(f"grid dimension {dim} not supported")
-        return {k: v.tolist() for k, v in box.items()}
+        """Cells of each face of the box as index lists (d rows), the interface of grid.py:34-86:
+        faces "bottom"/"top" (last axis at 0 / n-1), "left"/"right" (x) and, in 3-D,
+        "front"/"back" (y), in that order.  A face holds every cell whose coordinate on the face's
+        axis is fixed, enumerated in C order; with remove_edges the other coordinates run over
+        1 .. n-2 only."""
+        if self.dim not in _FACES:
+            raise ValueError(f"grid dimension {self.dim} not supported")
+        trim = 1 if remove_edges else 0
+        faces = {}
+        for name, axis, far in _FACES[self.dim]:
+            spans = [np.arange(trim, n - trim) for n in self.shape]
+            spans[axis] = np.array([self.shape[axis] - 1 if far else 0])
+            cells = np.stack(np.meshgrid(*spans, indexing="ij")).reshape(self.dim, -1)
+            faces[name] = cells.tolist()
+        return faces
+
+
+# (face name, axis held fixed, far side?) per dimension, in the order the faces are listed.
+_FACES = {
+    2: (("bottom", 1, False), ("top", 1, True), ("left", 0, False), ("right", 0, True)),
+    3: (("bottom", 2, False), ("top", 2, True), ("left", 0, False), ("right", 0, True), ("front", 1, False),
+        ("back", 1, True)),
+}
 
 
 class MI355XGrid(Grid):
