@@ -13,7 +13,8 @@
  * (card, nx, ny, 1).  `N = nx*ny*nz` is the number of cells.  Population l of cell
  * (x,y,z) lives at element  l*N + (x*ny + y)*nz + z.
  *
- * Grid limits: 1 <= nx <= 65535 and N <= 2^32 - 256; entry points that build cell lists
+ * Grid limits: 1 <= nx <= 65535, N <= 2^32 - 256, and at most 65535 launch tiles of y rows
+ * (ny / 4, ny / 2 or ny rows per tile for nz <= 64, <= 128 or more); entry points that build cell lists
  * (xlbh_compact_cells and the adjoint's boundary pass) need N < 2^31.  Planes above 4 GiB are
  * addressed with 64-bit offsets.
  *
@@ -278,8 +279,11 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
  *   bc_cells  : flat indices of every cell with a non-zero id, each once (xlbh_compact_cells with
  *               bc_id = -1), so the boundary pass runs over them only; NULL (n_bc_cells = 0)
  *               makes it scan the grid.  A list that misses boundary cells gives wrong gradients.
- * Other arguments as for xlbh_step.  Solid cells (id 255), which the forward step leaves
- * unwritten, pass nothing back.
+ * Other arguments as for xlbh_step.  Masks with solid cells (id 255) are refused: the forward
+ * step leaves those cells unwritten, so their populations depend on the caller's buffer schedule,
+ * which a per-step adjoint cannot differentiate.  The check reads bc_mask once and synchronises
+ * the stream: on every xlbh_adjoint_step call, and on xlbh_adjoint_run calls with XLBH_ADJ_BEGIN or
+ * without a workspace.
  * ------------------------------------------------------------------------------------- */
 int xlbh_adjoint_step(int lattice, int collision, int store, int compute,
                       const void* f_in, const void* lam_out, void* lam_in,

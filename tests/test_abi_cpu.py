@@ -58,8 +58,10 @@ def test_error_channel_and_validation_without_gpu():
     assert rc != 0 and b"lattice" in L.xlbh_last_error()
     # grid limits: empty extents, nx above the launch grid's z range, more than 2^32 - 256 cells
     a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    # (tiles of y rows: one row per tile once nz > 128, so ny = 65536 there is one tile too many)
     for (nx, ny, nz), msg in [((0, 8, 8), b"positive"), ((8, 8, -1), b"positive"), ((65536, 2, 2), b"65535"),
-                              ((65535, 65536, 2), b"2^32 - 256")]:
+                              ((65535, 65536, 2), b"2^32 - 256"), ((2, 65536, 129), b"tiles of y rows"),
+                              ((2, 262144, 64), b"tiles of y rows")]:
         rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
                          _lib.FORCE_NONE, None, None, nx, ny, nz, 1.0, 0.0, 0, None)
         assert rc != 0 and msg in L.xlbh_last_error(), (nx, ny, nz, L.xlbh_last_error())
@@ -85,3 +87,16 @@ def test_bc_desc_layout_matches_header():
 
     assert ctypes.sizeof(_lib.BcDesc) == 40
     assert _lib.BcDesc.rho.offset == 8 and _lib.BcDesc.u.offset == 16
+
+
+def test_largest_y_extent_passes_validation():
+    """65535 tiles of y rows are accepted: ny = 65535 with nz > 128 (one row per tile) and
+    ny = 4 * 65535 with nz <= 64 (four rows per tile); validation runs before any HIP call."""
+    from xlb_amd import _lib
+
+    L = _lib.lib()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    for nx, ny, nz in ((1, 65535, 129), (1, 4 * 65535, 2)):
+        rc = L.xlbh_run(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
+                        _lib.FORCE_NONE, None, None, nx, ny, nz, 1.0, 0.0, 0, 0, None)
+        assert rc == 0, L.xlbh_last_error()

@@ -272,6 +272,35 @@ def test_adjoint_rejects_bad_arguments():
                                       *masks, None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
                                       float(c3.omega), 0.0, _lib.stream_of(c3.f0))
     assert rc != 0 and b"D2Q9 and D3Q27" in _lib.lib().xlbh_last_error()
+    # a mask with a solid (255) cell: refused by both entry points, with and without a workspace,
+    # at an unaligned mask start (the scan's byte loop) and at an aligned one (its 16-byte loop)
+    c = Case("sphere3d", P.FP64FP64)
+    st = c.st
+    nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
+    lam_out, lam_in = c.W.clone(), torch.empty_like(c.W)
+    ncell = nx * ny * nz
+    for offset in (1, 0):
+        mask = torch.zeros(ncell + 16, dtype=torch.uint8, device="cuda")
+        view = mask[offset:offset + ncell]
+        view.copy_(c.bc_mask.reshape(-1))
+        view[ncell - 3] = 255
+        args = (st.lattice_id, st._collision_id(), st.precision_policy.store_format, st.compute_id)
+        rc = _lib.lib().xlbh_adjoint_step(*args, _lib.ptr(c.f0), _lib.ptr(lam_out), _lib.ptr(lam_in), _lib.ptr(view),
+                                          *masks[1:], None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
+                                          float(c.omega), st._collision_param(), _lib.stream_of(c.f0))
+        assert rc != 0 and b"solid" in _lib.lib().xlbh_last_error()
+        ws = st.adjoint_workspace(c.bc_mask)
+        for workspace, flags in ((None, 0), (ws, _lib.ADJ_BEGIN | _lib.ADJ_END)):
+            rc = _lib.lib().xlbh_adjoint_run(*args, _lib.ptr(c.f0), None, _lib.ptr(lam_out), _lib.ptr(lam_in),
+                                             _lib.ptr(view), *masks[1:], None, 0, force_mode, fu_ptr, _lib.ptr(ff),
+                                             None, nx, ny, nz, float(c.omega), st._collision_param(), 0, 1,
+                                             _lib.ptr(workspace), flags, _lib.stream_of(c.f0))
+            assert rc != 0 and b"solid" in _lib.lib().xlbh_last_error()
+        view[ncell - 3] = 0
+        rc = _lib.lib().xlbh_adjoint_step(*args, _lib.ptr(c.f0), _lib.ptr(lam_out), _lib.ptr(lam_in), _lib.ptr(view),
+                                          *masks[1:], None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
+                                          float(c.omega), st._collision_param(), _lib.stream_of(c.f0))
+        assert rc == 0, _lib.lib().xlbh_last_error()
 
 
 @pytest.mark.parametrize("name", ["tiny2d_channel", "tiny2d_regularized", "sphere3d", "do_nothing", "grads_sphere"])
@@ -314,3 +343,38 @@ def test_fused_and_split_reverse_agree(name):
     assert ((f_fused - f_split).norm() / f_split.norm()).item() <= 1e-12
     if g_split is not None:
         assert ((g_fused - g_split).norm() / g_split.norm()).item() <= 1e-12
+
+
+def test_rollout_keeps_its_own_force_field():
+    """Each rollout's backward pass recomputes its states with the force field of its own forward
+    pass: two rollouts with different fields run before either backward, and the caller's tensor
+    changed in place before the backward, give each the gradients of a rollout run alone, and the
+    force gradient matches central FD along a random direction."""
+    c = Case("channel2d_zouhe", P.FP64FP64, seed=6)
+    steps = 6
+    fields = (c.ff.clone(), -2.0 * c.ff.clone())
+
+    def start(ff):
+        f0 = c.f0.clone().requires_grad_(True)
+        g = ff.clone().requires_grad_(True)
+        fT = rollout(c.st, f0, c.bc_mask, c.missing, c.omega, steps, force_field=g, checkpoint_every=2)
+        return f0, g, c.loss(fT)
+
+    alone = []
+    for ff in fields:
+        f0, g, loss = start(ff)
+        loss.backward()
+        alone.append((f0.grad.clone(), g.grad.clone()))
+    runs = [start(ff) for ff in fields]
+    with torch.no_grad():
+        for _, g, _ in runs:
+            g.mul_(5.0)  # the caller changes its field tensors after the forward passes
+    for (f0, g, loss), (ref_f, ref_g) in zip(runs, alone):
+        loss.backward()
+        assert ((f0.grad - ref_f).norm() / ref_f.norm()).item() <= 1e-12
+        assert ((g.grad - ref_g).norm() / ref_g.norm()).item() <= 1e-12
+    v = torch.randn(c.ff.shape, generator=torch.Generator().manual_seed(7), dtype=torch.float64).cuda()
+    for ff, (_, ref_g) in zip(fields, alone):
+        fd = central(lambda t: c.loss_at(c.f0, steps, ff + t * v), 1e-6, False)
+        ad = float((ref_g * v).sum())
+        assert rel(ad, fd) <= FD_DIRECTION_TOL, f"adjoint {ad:.10e} vs FD {fd:.10e}"

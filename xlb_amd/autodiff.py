@@ -81,10 +81,14 @@ class _Rollout(torch.autograd.Function):
     def forward(ctx, f0, force_field, stepper, bc_mask, missing_mask, omega, num_steps, every, ckpt_mode, overlap,
                 fused):
         col = stepper.collision
+        field = None
         if force_field is not None:
             if not isinstance(col, ForcedCollision) or col.force_field is None:
                 raise ValueError("a differentiable force field needs a stepper built with force_field=...")
-            col.force_field = force_field.detach().to(stepper.compute_dtype).contiguous()
+            # A private copy: the backward pass recomputes this rollout's states with this field even
+            # if the caller changes its tensor, or runs another rollout on the stepper, in between.
+            field = force_field.detach().to(stepper.compute_dtype, copy=True).contiguous()
+            col.force_field = field
         w = torch.as_tensor(np.asarray(stepper.velocity_set.w), dtype=torch.float32, device=f0.device).reshape(-1, 1, 1, 1)
         a = f0.detach().to(stepper.store_dtype).clone().contiguous()
         b = torch.empty_like(a)
@@ -100,6 +104,7 @@ class _Rollout(torch.autograd.Function):
         ctx.ckpts = ckpts
         ctx.w = w
         ctx.has_force = force_field is not None
+        ctx.force_field = field
         ctx.force_shape = None if force_field is None else tuple(force_field.shape)
         ctx.force_dtype = None if force_field is None else force_field.dtype
         ctx.f0_dtype = f0.dtype
@@ -127,6 +132,10 @@ class _Rollout(torch.autograd.Function):
         main = torch.cuda.current_stream(lam_a.device)
         side = torch.cuda.Stream(device=lam_a.device) if ctx.overlap else None
         workspace = st.adjoint_workspace(bc_mask) if ctx.fused else None
+        col = st.collision
+        caller_field = getattr(col, "force_field", None)
+        if ctx.has_force:
+            col.force_field = ctx.force_field
 
         def record(idx):
             """Restore the checkpoint of segment idx and record its states into buffer idx % nbuf."""
@@ -136,28 +145,32 @@ class _Rollout(torch.autograd.Function):
             st.record(f_s, states[b], bc_mask, missing_mask, ctx.omega, min(k, n - s) - 1, timestep=s)
             return f_s
 
-        f_cur = record(0)
-        for idx, s in enumerate(starts):
-            length = min(k, n - s)
-            f_next = None
-            if idx + 1 < len(starts) and side is not None:
-                # the buffers of segment idx + 1 were last read by the sweep enqueued on main
-                side.wait_stream(main)
-                with torch.cuda.stream(side):
-                    f_next = record(idx + 1)
-            out = st.adjoint_run(f_cur, states[idx % nbuf], lam_a, lam_b, bc_mask, missing_mask, ctx.omega, length,
-                                 timestep=s, grad_force=grad_force, recompute=False, workspace=workspace,
-                                 begin=(idx == 0), end=(idx == len(starts) - 1))
-            if out is lam_b:
-                lam_a, lam_b = lam_b, lam_a
-            if idx + 1 < len(starts):
-                if side is not None:
-                    main.wait_stream(side)
-                else:
-                    f_next = record(idx + 1)
-            f_cur = f_next
+        try:
+            f_cur = record(0)
+            for idx, s in enumerate(starts):
+                length = min(k, n - s)
+                f_next = None
+                if idx + 1 < len(starts) and side is not None:
+                    # the buffers of segment idx + 1 were last read by the sweep enqueued on main
+                    side.wait_stream(main)
+                    with torch.cuda.stream(side):
+                        f_next = record(idx + 1)
+                out = st.adjoint_run(f_cur, states[idx % nbuf], lam_a, lam_b, bc_mask, missing_mask, ctx.omega,
+                                     length, timestep=s, grad_force=grad_force, recompute=False, workspace=workspace,
+                                     begin=(idx == 0), end=(idx == len(starts) - 1))
+                if out is lam_b:
+                    lam_a, lam_b = lam_b, lam_a
+                if idx + 1 < len(starts):
+                    if side is not None:
+                        main.wait_stream(side)
+                    else:
+                        f_next = record(idx + 1)
+                f_cur = f_next
+        finally:
+            if ctx.has_force:
+                col.force_field = caller_field
         if side is not None:
-            for t in states + [x for x in scratch if x is not None]:
+            for t in states + [x for x in scratch if x is not None] + ([ctx.force_field] if ctx.has_force else []):
                 t.record_stream(side)
         g_f0 = lam_a.to(ctx.f0_dtype)
         g_force = grad_force.to(ctx.force_dtype) if grad_force is not None else None

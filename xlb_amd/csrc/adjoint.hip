@@ -1,6 +1,8 @@
 // Reverse-mode entry points (xlbh_adjoint_step, xlbh_adjoint_run, xlbh_adjoint_workspace_bytes):
 // argument checks, the checkpoint-segment driver and the per-lattice dispatch of the kernels in
 // adjoint.hpp.
+#include <mutex>
+
 #include "adjoint.hpp"
 
 using namespace xlbh;
@@ -30,6 +32,58 @@ static int adjoint_validate(int lattice, int collision, int force_mode, const do
   XLBH_REQUIRE(force_mode != XLBH_FORCE_UNIFORM || force_uniform, "uniform force needs 3 host doubles");
   XLBH_REQUIRE(force_mode != XLBH_FORCE_FIELD || force_field, "force field pointer is null");
   XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 31), "the adjoint supports fewer than 2^31 cells");
+  return 0;
+}
+
+// Solid cells (id 255) are refused by the reverse-mode entry points: the forward step leaves them
+// unwritten, so in a rollout their populations are whatever the output buffer held two steps
+// earlier -- a dependence on the buffer schedule that no per-step adjoint can express, and which
+// xlbh_record (whose state buffers are never written there) cannot reproduce.  The scan below reads
+// the mask once and reports whether any byte is 255; the flag lives in a per-device __device__
+// word, cleared by a one-thread kernel before each scan.
+static __device__ int g_solid_found;
+
+__global__ void k_clear_solid_flag(int* flag) { *flag = 0; }
+
+__global__ __launch_bounds__(256) void k_find_solid(const uint8_t* mask, uint64_t n, int* flag) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n16 = ((uintptr_t)mask & 15) == 0 ? n / 16 : 0;
+  const uint4* v = reinterpret_cast<const uint4*>(mask);
+  bool found = false;
+  for (uint64_t i = tid; i < n16; i += stride) {
+    const uint4 q = v[i];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t inv = ~w[k];  // a byte of w is 0xff <=> that byte of inv is zero
+      found |= ((inv - 0x01010101u) & ~inv & 0x80808080u) != 0;
+    }
+  }
+  for (uint64_t i = n16 * 16 + tid; i < n; i += stride) found |= mask[i] == 255;
+  if (found) *flag = 1;
+}
+
+static int refuse_solid_cells(const uint8_t* bc_mask, int nx, int ny, int nz, hipStream_t s) {
+  if (!bc_mask) return 0;
+  static std::mutex serial;  // one scan at a time: the flag word is shared by every caller on a device
+  std::lock_guard<std::mutex> hold(serial);
+  int dev = 0;
+  XLBH_HIP_CHECK(hipGetDevice(&dev));
+  static int* flag_of[64] = {};
+  XLBH_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+  if (!flag_of[dev]) XLBH_HIP_CHECK(hipGetSymbolAddress(reinterpret_cast<void**>(&flag_of[dev]), HIP_SYMBOL(g_solid_found)));
+  int* flag = flag_of[dev];
+  const uint64_t n = (uint64_t)nx * ny * nz;
+  hipLaunchKernelGGL(k_clear_solid_flag, dim3(1), dim3(1), 0, s, flag);
+  XLBH_LAUNCH_CHECK();
+  const uint64_t blocks = std::min<uint64_t>((n / 16 + 255) / 256 + 1, 4096);
+  hipLaunchKernelGGL(k_find_solid, dim3((unsigned)blocks), dim3(256), 0, s, bc_mask, n, flag);
+  XLBH_LAUNCH_CHECK();
+  int found = 0;
+  XLBH_HIP_CHECK(hipMemcpyAsync(&found, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  XLBH_HIP_CHECK(hipStreamSynchronize(s));
+  XLBH_REQUIRE(!found, "the adjoint does not support solid (id 255) cells");
   return 0;
 }
 
@@ -81,6 +135,10 @@ extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compu
   XLBH_REQUIRE(lam_a != lam_b, "lam_a must not alias lam_b");
   XLBH_REQUIRE(n_steps == 1 || states, "adjoint_run needs a state workspace for n_steps > 1");
   XLBH_REQUIRE(valid_store(store), "unknown store format");
+  // the mask scan synchronises the stream: once per sweep (XLBH_ADJ_BEGIN) with a workspace,
+  // every call without one
+  if (!workspace || (flags & XLBH_ADJ_BEGIN))
+    if (int e = refuse_solid_cells(bc_mask, nx, ny, nz, (hipStream_t)stream)) return e;
   const size_t field_bytes = (size_t)lattice_q(lattice) * (size_t)nx * ny * nz * store_bytes(store);
   auto state = [&](int j) -> const void* {
     return j == 0 ? f_ckpt : static_cast<const void*>(static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes);
@@ -156,6 +214,7 @@ extern "C" int xlbh_adjoint_step(int lattice, int collision, int store, int comp
   XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
   XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
   XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
+  if (int e = refuse_solid_cells(bc_mask, nx, ny, nz, (hipStream_t)stream)) return e;
   const AdjParams p = adjoint_params(collision, collision_param, f_in, lam_out, lam_in, bc_mask, missing_bits, aux,
                                      bc_table, bc_cells, n_bc_cells, force_mode, force_uniform, force_field,
                                      grad_force, nx, ny, nz, omega);

@@ -748,11 +748,18 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
   asm volatile("" ::: "memory");
   C* li = static_cast<C*>(p.lam_in);
   if (id == 255) {
-    // solid cells are not updated by the forward step: nothing flows back through their pulls
+    // solid cells are not updated by the forward step: nothing flows back through their pulls.
+    // (The entry points refuse masks with solid cells; this path keeps the kernel's buffers
+    // consistent regardless: deltas addressed to the cell are consumed like anywhere else.)
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
       const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
       li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = C(0);
+    }
+    if (nbf) {
+      C* dni = static_cast<C*>(p.d_nb_in);
+#pragma unroll
+      for (int l = 0; l < Q; ++l) dni[(size_t)l * p.N + cell] = C(0);
     }
     return;
   }

@@ -18,6 +18,9 @@ int check_grid(int lattice, int nx, int ny, int nz) {
   // 1-D launches round the cell count up to whole 256-thread blocks in 32-bit arithmetic.
   XLBH_REQUIRE((int64_t)nx * ny * nz <= ((int64_t)1 << 32) - 256, "more than 2^32 - 256 cells per field");
   XLBH_REQUIRE(nx <= 65535, "nx must be <= 65535");
+  // the sweep's grid: (z tiles, y tiles, x) in 3-D, (y tiles, x tiles, 1) in 2-D
+  const dim3 g = sweep_grid(make_sweep(nx, ny, nz));
+  XLBH_REQUIRE(g.y <= 65535, "ny too large for the sweep's launch grid (more than 65535 tiles of y rows)");
   return 0;
 }
 
