@@ -275,8 +275,11 @@ LARGE_CASES = {
     "c2_sphere": ("D3Q27", (256, 128, 128), "KBC", 1.0 / (3 * 0.04 * 24 / 200 + 0.5), "rest"),
     # C3: periodic D3Q19 BGK Taylor-Green vortex 512x256x256, u0 = 0.04, nu = 0.01
     "c3_tgv": ("D3Q19", (512, 256, 256), "BGK", 1.0 / (3 * 0.01 + 0.5), "tgv"),
+    # the benchmark workload (mlups_3d.py:45-90) at edge 256, and at 64
+    "mlups_cavity256": ("D3Q19", (256, 256, 256), "BGK", 1.0, "rest"),
+    "mlups_cavity64": ("D3Q19", (64, 64, 64), "BGK", 1.0, "rest"),
 }
-BC_SET = {"sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "c3_tgv": "periodic", "cavity_smagorinsky": "cavity_halfway",
+BC_SET = {"mlups_cavity256": "cavity_fullway", "mlups_cavity64": "cavity_fullway", "sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "c3_tgv": "periodic", "cavity_smagorinsky": "cavity_halfway",
           "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d",
           "sphere3d_grads_bgk": "sphere3d_grads", "tiny_periodic3d": "periodic", "tiny_periodic2d": "periodic",
           "tiny_periodic_kbc": "periodic", "tiny_cavity2d": "cavity2d", "tiny_cavity3d": "cavity_fullway"}

@@ -1,10 +1,12 @@
 """GPU parity of the fused stepper (xlbh_step / xlbh_run) against the CPU oracle.
 
 Tolerances, per precision:
-* FP32 store and compute: max |drho|, |du| <= 1e-5 after N steps, and max |df| <= max(2e-6,
-  2 x the oracle's own FP32 error against an FP64 oracle run of the same case) -- FP32 rounding
-  (FMA contraction on the GPU, none in the oracle) is amplified differently by each case, most
-  by KBC's entropic gamma near equilibrium, and the GPU must be as accurate as an FP32 CPU code.
+* FP32 store and compute: max |drho|, |du| <= 1e-5 after N steps, with no exception, and max |df|
+  <= 2e-6 or the fixed per-case bound in F32_DF_BOUND.  Those bounds hold for cases whose FP32
+  rounding (FMA contraction on the GPU, none in the oracle) is amplified, most by KBC's entropic
+  gamma near equilibrium; each is about twice the largest |df| measured on the GPU.  Where |df|
+  exceeds 2e-6, the oracle's own FP32 error against an FP64 oracle run of the same case is printed
+  beside it as a diagnostic (the noise floor an FP32 CPU code reaches).
 * FP64: max |df| <= 1e-12.
 * 16-bit storage: both sides round to 16 bits independently, so stored values may differ by a
   few units in the last place; max distance <= 4 ulps of the stored value, and rho / u bounds
@@ -34,6 +36,10 @@ TOL = {
     P.FP32BF16_SHIFTED: (None, 2e-4),
 }
 
+# FP32 cases whose max |df| exceeds 2e-6 (amplified rounding): fixed bounds, about twice the
+# largest value measured on the GPU (case: bound).
+F32_DF_BOUND = {}
+
 SIXTEEN = (orc.F16, orc.BF16, orc.F16_SHIFTED, orc.BF16_SHIFTED)
 
 
@@ -50,7 +56,7 @@ def stored_ulp(raw, fmt):
     return 2.0 ** (np.floor(np.log2(m)) - mant) if m > 0 else 0.0
 
 
-def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None):
+def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None, noise_floor=True):
     pair.step_product(steps, use_run=use_run)
     pair.step_oracle(steps)
     fa, fb = pair.product_f(), pair.oracle_f()
@@ -62,6 +68,7 @@ def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None):
     rb, ub = moments(fb, pair.vs.c)
     dr = np.abs(ra - rb)[~solid].max()
     du = np.abs(ua - ub)[:, ~solid].max()
+    print(f"parity {pair.name} {pair.policy.name} {steps} steps: max|df| {df:.2e} max|drho| {dr:.2e} max|du| {du:.2e}")
     if pair.fmt in SIXTEEN:
         ulp = stored_ulp(pair.o_f0, pair.fmt)
         assert df <= ulps * ulp, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > {ulps} ulps ({ulps * ulp:.3e})"
@@ -70,13 +77,13 @@ def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None):
             assert mean <= mean_ulps * ulp, f"{pair.name} {pair.policy}: mean|df| = {mean:.3e} > {mean_ulps} ulps"
         # rho and u sum Q stored values, each of which may be a few ulps off
         tm = max(tm, 4 * ulp * np.sqrt(pair.vs.q))
-    elif pair.policy == P.FP32FP32 and (df > tf or dr > tm or du > tm):
-        # FP32 noise floor of this case: the oracle's own FP32 error against an FP64 oracle run
-        ref = pair.oracle_reference_f(steps, P.FP64FP64)
-        rr, ur = moments(ref, pair.vs.c)
-        tf = max(tf, 2 * np.abs(ref - fb).max())
-        tm = max(tm, 2 * np.abs(rr - rb)[~solid].max(), 2 * np.abs(ur - ub)[:, ~solid].max())
-    if pair.fmt not in SIXTEEN:
+    else:
+        if pair.policy == P.FP32FP32 and tol is None:
+            tf = F32_DF_BOUND.get(pair.name, tf)
+        if pair.policy == P.FP32FP32 and df > 2e-6 and noise_floor:
+            ref = pair.oracle_reference_f(steps, P.FP64FP64)
+            print(f"  noise floor {pair.name}: the oracle's own FP32 max|df| against its FP64 run "
+                  f"{np.abs(ref - fb).max():.2e}")
         assert df <= tf, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > {tf:.1e}"
     assert dr <= tm and du <= tm, f"{pair.name} {pair.policy}: max|drho| = {dr:.3e}, max|du| = {du:.3e} > {tm:.1e}"
     return df, dr, du
@@ -103,25 +110,35 @@ def test_c1_cavity64_1000_steps_matches_oracle(policy):
     assert dr <= bound and du <= bound
 
 
-def test_c2_full_size_matches_oracle():
-    """C2 at its full size (D3Q27 KBC wind tunnel 256x128x128, sphere r = 12): 20 FP32 steps
-    against the oracle, with the stepper tolerances above."""
-    pair = Pair("c2_sphere", P.FP32FP32)
-    df, dr, du = compare(pair, 20)
-    print(f"C2 full size, 20 steps: max|df| {df:.2e} max|drho| {dr:.2e} max|du| {du:.2e}")
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+def test_c2_500_steps_matches_oracle(policy):
+    """C2 at its full size (D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, Re = 200): 500 steps
+    (the flow around the sphere is developing) against the oracle in FP32 and FP64, with the
+    stepper tolerances above (rho, u <= 1e-5 in FP32; f <= 1e-12 in FP64)."""
+    compare(Pair("c2_sphere", policy), 500, use_run=True, noise_floor=False)
 
 
-@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP32FP16_SHIFTED])
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED])
 def test_c3_full_size_matches_oracle(policy):
     """C3 at its full size (periodic Taylor-Green vortex 512x256x256): 10 steps against the
-    oracle in FP32 and in shifted FP16 storage.  For FP16 the two sides round independently (FMA
-    contraction on the GPU only) and a one-ulp difference is carried and mixed by later steps; over
-    6.4e8 stored values the worst one reaches a few ulps (measured 4.25), so the elementwise bound
-    is 8 ulps of the field's largest stored magnitude, with the mean difference below a quarter of
-    that ulp (measured 0.14: most values are far smaller than the largest)."""
-    pair = Pair("c3_tgv", policy)
-    df, dr, du = compare(pair, 10, ulps=8, mean_ulps=0.25)
-    print(f"C3 full size {policy.name}, 10 steps: max|df| {df:.2e} max|drho| {dr:.2e} max|du| {du:.2e}")
+    oracle in FP32, FP64 and shifted FP16 / BF16 storage.  For 16-bit storage the two sides round
+    independently (FMA contraction on the GPU only) and a one-ulp difference is carried and mixed by
+    later steps; over 6.4e8 stored values the worst one reaches a few ulps (measured 4.25 in FP16),
+    so the elementwise bound is 8 ulps of the field's largest stored magnitude, with the mean
+    difference below a quarter of that ulp (measured 0.14: most values are far smaller than the
+    largest)."""
+    compare(Pair("c3_tgv", policy), 10, ulps=8, mean_ulps=0.25, noise_floor=False)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+def test_mlups_cavity256_matches_oracle(policy):
+    """The benchmark's own workload (mlups_3d.py:45-90 at edge 256: D3Q19 BGK, EquilibriumBC lid
+    u = (0.02, 0, 0) on the top face without edges, FullwayBounceBackBC on the other five faces,
+    omega = 1, rest start): 30 steps through the C loop against the oracle."""
+    compare(Pair("mlups_cavity256", policy), 30, use_run=True, noise_floor=False)
 
 
 @pytest.mark.parametrize("name", ["periodic", "cavity_fullway", "channel2d_zouhe", "channel2d_regularized", "sphere3d",
@@ -154,23 +171,7 @@ def test_run_equals_python_loop(name):
 
 def test_mlups_cavity_64_fp32():
     """The reference benchmark's configuration (mlups_3d.py:45-84) at 64^3, 200 steps."""
-    from lbm_cases import CASES as C
-
-    C["cavity_fullway_64"] = ("D3Q19", (64, 64, 64), "BGK", 1.0, "rest")
-    try:
-        import lbm_cases
-
-        orig = lbm_cases.make_bcs
-
-        def mk(name, grid, xa_):
-            return orig("cavity_fullway" if name == "cavity_fullway_64" else name, grid, xa_)
-
-        lbm_cases.make_bcs = mk
-        pair = Pair("cavity_fullway_64", P.FP32FP32)
-        df, dr, du = compare(pair, 200, tol=(5e-6, 1e-5))
-    finally:
-        lbm_cases.make_bcs = orig
-        C.pop("cavity_fullway_64", None)
+    compare(Pair("mlups_cavity64", P.FP32FP32), 200)
 
 
 def test_steps_are_deterministic():

@@ -1,12 +1,18 @@
 // FP64 reductions over the lattice: momentum-exchange force and global mass/momentum.
 // Two passes, both in a fixed order, so results are bitwise reproducible run to run:
-// pass 1 reduces each 256-cell block (wave64 shuffle tree, then the 4 wave sums in LDS) into a
-// per-block partial; pass 2 has one block sum the partials in a fixed strided order + tree.
+// pass 1 reduces each block's kRedCells cells (each thread sums kRedRun cells strided by 256,
+// then a wave64 shuffle tree and the 4 wave sums in LDS) into a per-block partial; pass 2 has one
+// block sum the partials in a fixed strided order + tree.  With 4096 cells per block a 256^3 grid
+// leaves 4096 partials, which one block sums in a few microseconds.
 #include "physics.hpp"
 
 namespace xlbh {
 
 constexpr int kRedWidth = 4;  // doubles per partial
+constexpr int kRedRun = 16;   // cells per thread
+constexpr uint32_t kRedCells = 256u * kRedRun;
+
+inline uint32_t red_blocks(uint32_t n_cells) { return (n_cells + kRedCells - 1) / kRedCells; }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -41,68 +47,85 @@ __global__ __launch_bounds__(256) void k_final_sum(const double* __restrict__ pa
     for (int k = 0; k < width; ++k) out[k] = res[k];
 }
 
+// One boundary cell's momentum exchange, added to m: the post-collision populations f_pc, the
+// re-pulled post-stream populations with the BC's streaming functional applied (f_ps), and the sum
+// over missing l of c_{opp l} (f_pc[opp l] + f_ps[l]) (momentum_transfer.py:137-160).
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ void momentum_cell(const typename Fmt<FMT>::raw* __restrict__ fin, uint32_t mm,
+                                              const C* aux, int kind, const xlbh_bc_desc& desc, int nx, int ny,
+                                              int nz, uint32_t N, uint32_t cell, double (&m)[kRedWidth]) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  const int z = cell % nz;
+  const int y = (cell / nz) % ny;
+  const int x = cell / ((uint32_t)nz * ny);
+  C fpc[Q], fps[Q];
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    fpc[l] = ld_pop<FMT, C>(fin + (size_t)l * N + cell, T.w[l]);
+    const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
+    fps[l] = ld_pop<FMT, C>(fin + (size_t)l * N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
+  }
+  if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+      if (bit(mm, l)) fps[l] = fpc[T.opp[l]];
+  } else if (kind == XLBH_BC_FULLWAY_BB) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fps[l] = fpc[T.opp[l]];
+  } else if (kind == XLBH_BC_DO_NOTHING) {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fps[l] = fpc[l];
+  } else if (kind == XLBH_BC_EQUILIBRIUM) {
+    const C rb = (C)desc.rho;
+    const C ub[3] = {(C)desc.u[0], (C)desc.u[1], (C)desc.u[2]};
+    const C us = usqr15<LAT>(ub);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fps[l] = feq_l<LAT>(l, rb, ub, us);
+  } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
+    zouhe_cell<LAT>(fps, mm, aux[cell], kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
+                    kind >= XLBH_BC_REGULARIZED_VELOCITY);
+  } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
+    grads_cell<LAT>(fps, mm, [&](int l) { return fpc[l]; });
+  }
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) {
+    C md = C(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      if (bit(mm, l)) {
+        const C phi = fpc[T.opp[l]] + fps[l];
+        if (T.c[d][T.opp[l]] == 1) md += phi;
+        else if (T.c[d][T.opp[l]] == -1) md -= phi;
+      }
+    }
+    m[d] += (double)md;
+  }
+}
+
 // Momentum exchange (momentum_transfer.py:108-160): at cells with bc_mask == id whose rest
 // direction is not missing, re-pull the post-stream populations, apply the BC's streaming
-// functional, and sum over missing l of c_{opp l} (f_pc[opp l] + f_ps[l]).
+// functional, and sum over missing l of c_{opp l} (f_pc[opp l] + f_ps[l]).  The kRedRun ids of a
+// thread are loaded together before any cell is processed.
 template <int LAT, int FMT, class C>
 __global__ __launch_bounds__(256) void k_momentum(const typename Fmt<FMT>::raw* __restrict__ fin,
                                                   const uint8_t* __restrict__ bc_mask,
                                                   const uint32_t* __restrict__ missing, const C* aux, int kind,
                                                   int id, xlbh_bc_desc desc, int nx, int ny, int nz, uint32_t N,
                                                   double* __restrict__ part) {
-  constexpr LatTables T = make_tables(LAT);
-  constexpr int Q = T.q;
-  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t base = blockIdx.x * kRedCells + threadIdx.x;
+  uint8_t ids[kRedRun];
+#pragma unroll
+  for (int k = 0; k < kRedRun; ++k) {
+    const uint32_t cell = base + 256u * k;
+    ids[k] = cell < N ? bc_mask[cell] : 0;
+  }
   double m[kRedWidth] = {0, 0, 0, 0};
-  if (cell < N && bc_mask[cell] == id) {
+  for (int k = 0; k < kRedRun; ++k) {
+    const uint32_t cell = base + 256u * k;
+    if (ids[k] != id) continue;
     const uint32_t mm = missing[cell];
-    if (!bit(mm, 0)) {
-      const int z = cell % nz;
-      const int y = (cell / nz) % ny;
-      const int x = cell / ((uint32_t)nz * ny);
-      C fpc[Q], fps[Q];
-#pragma unroll
-      for (int l = 0; l < Q; ++l) {
-        fpc[l] = ld_pop<FMT, C>(fin + (size_t)l * N + cell, T.w[l]);
-        const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
-        fps[l] = ld_pop<FMT, C>(fin + (size_t)l * N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
-      }
-      if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l)
-          if (bit(mm, l)) fps[l] = fpc[T.opp[l]];
-      } else if (kind == XLBH_BC_FULLWAY_BB) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) fps[l] = fpc[T.opp[l]];
-      } else if (kind == XLBH_BC_DO_NOTHING) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) fps[l] = fpc[l];
-      } else if (kind == XLBH_BC_EQUILIBRIUM) {
-        const C rb = (C)desc.rho;
-        const C ub[3] = {(C)desc.u[0], (C)desc.u[1], (C)desc.u[2]};
-        const C us = usqr15<LAT>(ub);
-#pragma unroll
-        for (int l = 0; l < Q; ++l) fps[l] = feq_l<LAT>(l, rb, ub, us);
-      } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
-        zouhe_cell<LAT>(fps, mm, aux[cell], kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
-                        kind >= XLBH_BC_REGULARIZED_VELOCITY);
-      } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
-        grads_cell<LAT>(fps, mm, [&](int l) { return fpc[l]; });
-      }
-#pragma unroll
-      for (int d = 0; d < T.d; ++d) {
-        C md = C(0);
-#pragma unroll
-        for (int l = 0; l < Q; ++l) {
-          if (bit(mm, l)) {
-            const C phi = fpc[T.opp[l]] + fps[l];
-            if (T.c[d][T.opp[l]] == 1) md += phi;
-            else if (T.c[d][T.opp[l]] == -1) md -= phi;
-          }
-        }
-        m[d] = (double)md;
-      }
-    }
+    if (!bit(mm, 0)) momentum_cell<LAT, FMT, C>(fin, mm, aux, kind, desc, nx, ny, nz, N, cell, m);
   }
   block_sum<kRedWidth>(m, part + (size_t)blockIdx.x * kRedWidth);
 }
@@ -113,15 +136,17 @@ __global__ __launch_bounds__(256) void k_mass_momentum(const typename Fmt<FMT>::
                                                        const uint8_t* __restrict__ bc_mask, uint32_t N,
                                                        double* __restrict__ part) {
   constexpr LatTables T = make_tables(LAT);
-  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
   double v[kRedWidth] = {0, 0, 0, 0};
-  if (cell < N && (bc_mask == nullptr || bc_mask[cell] != 255)) {
+  for (int k = 0; k < kRedRun; ++k) {
+    const uint32_t cell = blockIdx.x * kRedCells + threadIdx.x + 256u * k;
+    if (cell < N && (bc_mask == nullptr || bc_mask[cell] != 255)) {
 #pragma unroll
-    for (int l = 0; l < T.q; ++l) {
-      const double f = ld_pop<FMT, double>(fin + (size_t)l * N + cell, T.w[l]);
-      v[0] += f;
+      for (int l = 0; l < T.q; ++l) {
+        const double f = ld_pop<FMT, double>(fin + (size_t)l * N + cell, T.w[l]);
+        v[0] += f;
 #pragma unroll
-      for (int d = 0; d < T.d; ++d) v[1 + d] += (double)T.c[d][l] * f;
+        for (int d = 0; d < T.d; ++d) v[1 + d] += (double)T.c[d][l] * f;
+      }
     }
   }
   block_sum<kRedWidth>(v, part + (size_t)blockIdx.x * kRedWidth);
@@ -147,7 +172,7 @@ int lat_dispatch(int lat, Fn&& fn) {
 extern "C" {
 
 int64_t xlbh_reduce_workspace_bytes(int64_t n_cells) {
-  return ((n_cells + 255) / 256) * kRedWidth * (int64_t)sizeof(double);
+  return ((n_cells + kRedCells - 1) / kRedCells) * kRedWidth * (int64_t)sizeof(double);
 }
 
 int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
@@ -162,7 +187,7 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
   xlbh_bc_desc d{};
   if (desc) d = *desc;
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
-  const int nb = (int)((N + 255) / 256);
+  const int nb = (int)red_blocks(N);
   double* part = static_cast<double*>(workspace);
   hipStream_t s = (hipStream_t)stream;
   return lat_dispatch(lattice, [&](auto L) -> int {
@@ -205,7 +230,7 @@ int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_
   XLBH_REQUIRE(f && out && workspace, "mass_momentum: null argument");
   XLBH_REQUIRE(valid_store(store), "unknown store format");
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
-  const int nb = (int)((N + 255) / 256);
+  const int nb = (int)red_blocks(N);
   double* part = static_cast<double*>(workspace);
   hipStream_t s = (hipStream_t)stream;
   return lat_dispatch(lattice, [&](auto L) -> int {
