@@ -38,6 +38,7 @@ Usage:
 import argparse
 import csv
 import json
+import re
 import os
 import sys
 import time
@@ -110,14 +111,17 @@ def decoded_velocity(torch, f, policy, vs, chunk=32):
     """Velocity (d, ...) in float64 of stored populations, by explicit sums over the directions in
     x-chunks (no BLAS call: torch's tensordot on fields of this size, or on strided views into
     them, returned wrong moments on ROCm)."""
+    import xlb_amd as xa
+
     c = np.asarray(vs.c, dtype=np.float64)
     w = np.asarray(vs.w, dtype=np.float64)
     u = torch.empty((vs.d,) + tuple(f.shape[1:]), dtype=torch.float64, device=f.device)
+    dec = xa.grid.decode_populations(f, vs.w, block=True) if policy.block else None
     for i in range(0, f.shape[1], chunk):
         rho = torch.zeros_like(u[0, i:i + chunk])
         mom = torch.zeros_like(u[:, i:i + chunk])
         for l in range(vs.q):
-            fl = f[l, i:i + chunk].double()
+            fl = dec[l, i:i + chunk] if dec is not None else f[l, i:i + chunk].double()
             if policy.shifted:
                 fl = fl + w[l]
             rho += fl
@@ -135,7 +139,8 @@ def precision_leg(xa, torch, shape=(512, 256, 256), steps=2000):
     P = xa.PrecisionPolicy
     out, u64, ke0 = {}, None, None
     cells = int(np.prod(shape))
-    for pol in (P.FP64FP64, P.FP32FP32, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16, P.FP32BF16):
+    for pol in (P.FP64FP64, P.FP32FP32, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16,
+                P.FP32BF16):
         wl = taylor_green_3d(shape, pol)
         vs = wl.stepper.velocity_set
         if ke0 is None:
@@ -167,7 +172,7 @@ def precision_viscosity_sweep(xa, torch, shape=(512, 256, 256), steps=2000, nus=
     for nu in nus:
         row = {"tau": 3 * nu + 0.5}
         u64 = None
-        for pol in (P.FP64FP64, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED):
+        for pol in (P.FP64FP64, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED):
             wl = taylor_green_3d(shape, pol, nu=nu)
             vs = wl.stepper.velocity_set
             _, _, f0, f1 = time_run(torch, wl, steps, 0)
@@ -194,7 +199,8 @@ def headline_precision_leg(xa, torch, n=256, steps=1000):
 
     P = xa.PrecisionPolicy
     out, u64 = {}, None
-    for pol in (P.FP64FP64, P.FP32FP32, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16, P.FP32BF16):
+    for pol in (P.FP64FP64, P.FP32FP32, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16,
+                P.FP32BF16):
         wl = lid_driven_cavity_3d(n, pol)
         f0, _ = wl.stepper.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, steps)
         u = decoded_velocity(torch, f0, pol, wl.stepper.velocity_set)
@@ -470,6 +476,33 @@ def copy_leg(torch, nbytes=2 * 19 * 4 * 256**3 // 2, reps=20):
     return dict(bytes=2 * nbytes, gbs=2 * nbytes / s / 1e9)
 
 
+def pull_pattern_leg(xa, torch, policy, n=256, reps=50):
+    """The step kernel's memory pattern alone: the Stream operator (the same Q pulled loads, Q
+    non-temporal stores, addressing, launch shape and occupancy as the fused step, no arithmetic)
+    on the headline cavity's fields.  Its rate is the ceiling the step's access pattern allows."""
+    from xlb_amd.workloads import lid_driven_cavity_3d
+
+    wl = lid_driven_cavity_3d(n, policy)
+    stream = xa.operator.stream.Stream(wl.stepper.velocity_set, policy, wl.stepper.compute_backend)
+    a, b = wl.f_0, wl.f_1
+    for _ in range(5):
+        stream(a, b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        stream(a, b)
+        a, b = b, a
+    ev1.record()
+    torch.cuda.synchronize()
+    s = ev0.elapsed_time(ev1) / 1e3 / reps
+    bpl = 2 * wl.stepper.velocity_set.q * store_bytes(torch, a)
+    del wl, a, b
+    torch.cuda.empty_cache()
+    return dict(policy=policy.name, kernel_ms=s * 1e3, gbs=bpl * n**3 / s / 1e9, mlups=n**3 / s / 1e6)
+
+
 def c1_leg(xa, torch, with_cpu=True):
     """C1: 64^3 lid-driven cavity, halfway walls, lid u = (0.05, 0, 0), Re = 100, 1000 steps, FP32:
     GPU rate, and the oracle's rate on the host cores for the same 1000 steps."""
@@ -603,10 +636,11 @@ def _read_pmc(path, counter):
 
 
 def summarize_pmc(fetch_csv, write_csv, n):
-    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the step kernel, each scaled by what the same
-    counter reads for the 1 GiB calibration copy (the first three copy dispatches of the profiled
-    process, issued by ``profile_pass`` before anything else is copied) ->
-    profiles/traffic_step_d3q19_cavity{n}.json."""
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of each step kernel profiled, each scaled by what
+    the same counter reads for the 1 GiB calibration copy (the first three copy dispatches of the
+    profiled process, issued by ``profile_pass`` before anything else is copied).  The FP32 step
+    kernel's figures go to profiles/traffic_step_d3q19_cavity{n}.json (the headline line's
+    ``traffic``); a 16-bit kernel's to profiles/traffic_step16_<store>_d3q19_cavity{n}.json."""
     fetch, write = _read_pmc(fetch_csv, "FETCH_SIZE"), _read_pmc(write_csv, "WRITE_SIZE")
 
     def calib(rows):
@@ -615,30 +649,34 @@ def summarize_pmc(fetch_csv, write_csv, n):
             raise SystemExit("calibration copy dispatches not found")
         return float(np.median(vals))
 
-    def step(rows):
-        ks = sorted({k for _, k, _ in rows if "k_step<" in k})
-        if len(ks) != 1:
-            raise SystemExit(f"expected one step kernel, found {ks}")
-        return ks[0], float(np.median([v for _, k, v in rows if k == ks[0]]))
-
+    kernels = sorted({k for _, k, _ in fetch if "k_step" in k})
+    if not kernels:
+        raise SystemExit("no step kernel in the counter file")
     copy_fetch, copy_write = calib(fetch), calib(write)
-    kernel, step_fetch = step(fetch)
-    _, step_write = step(write)
     kib = COPY_CALIBRATION_BYTES / 1024.0
     f_scale, w_scale = kib / copy_fetch, kib / copy_write
-    traffic = (step_fetch * f_scale + step_write * w_scale) * 1024.0
-    algorithmic = 2 * 19 * 4 * n**3
-    out = dict(kernel=kernel, copy_bytes=COPY_CALIBRATION_BYTES,
-               copy_fetch_kib=copy_fetch, copy_write_kib=copy_write, fetch_scale=f_scale, write_scale=w_scale,
-               step_fetch_kib=step_fetch, step_write_kib=step_write,
-               fetch_bytes=step_fetch * f_scale * 1024.0, write_bytes=step_write * w_scale * 1024.0,
-               bytes_per_launch=traffic, algorithmic_bytes_per_launch=algorithmic,
-               traffic_over_algorithmic=traffic / algorithmic)
-    path = TRAFFIC_JSON.format(n=n)
-    with open(path, "w") as fh:
-        json.dump(out, fh, indent=1)
-    print(json.dumps(out, indent=1))
-    return out
+    store_names = {0: "f32", 1: "f64", 2: "f16", 3: "bf16", 4: "f16_shifted", 5: "bf16_shifted", 6: "i16_block"}
+    results = {}
+    for kernel in kernels:
+        store = int(re.search(r"k_step\w*<\d+, \d+, (\d+),", kernel).group(1))
+        sbytes = {0: 4, 1: 8}.get(store, 2)
+        step_fetch = float(np.median([v for _, k, v in fetch if k == kernel]))
+        step_write = float(np.median([v for _, k, v in write if k == kernel]))
+        traffic = (step_fetch * f_scale + step_write * w_scale) * 1024.0
+        algorithmic = 2 * 19 * sbytes * n**3
+        out = dict(kernel=kernel, copy_bytes=COPY_CALIBRATION_BYTES,
+                   copy_fetch_kib=copy_fetch, copy_write_kib=copy_write, fetch_scale=f_scale, write_scale=w_scale,
+                   step_fetch_kib=step_fetch, step_write_kib=step_write,
+                   fetch_bytes=step_fetch * f_scale * 1024.0, write_bytes=step_write * w_scale * 1024.0,
+                   bytes_per_launch=traffic, algorithmic_bytes_per_launch=algorithmic,
+                   traffic_over_algorithmic=traffic / algorithmic)
+        path = TRAFFIC_JSON.format(n=n) if store == 0 else os.path.join(
+            os.path.dirname(TRAFFIC_JSON), f"traffic_step16_{store_names[store]}_d3q19_cavity{n}.json")
+        with open(path, "w") as fh:
+            json.dump(out, fh, indent=1)
+        results[kernel] = out
+    print(json.dumps(results, indent=1))
+    return results
 
 
 def main():
@@ -694,13 +732,16 @@ def main():
     log(f"bench: fp32 {fp32['mlups']:.0f} MLUPS ({fp32['achieved_gbs']:.0f} GB/s)")
     store16 = {}
     if not args.no_store16:
-        for pol in (xa.PrecisionPolicy.FP32FP16, xa.PrecisionPolicy.FP32FP16_SHIFTED, xa.PrecisionPolicy.FP32BF16_SHIFTED):
+        for pol in (xa.PrecisionPolicy.FP32I16_BLOCK, xa.PrecisionPolicy.FP32FP16, xa.PrecisionPolicy.FP32FP16_SHIFTED,
+                    xa.PrecisionPolicy.FP32BF16_SHIFTED):
             r = headline_leg(xa, torch, pol, n, args.steps, args.warmup, python_loop=False)
             log(f"bench: {pol.name} {r['mlups']:.0f} MLUPS ({r['achieved_gbs']:.0f} GB/s)")
             store16[pol.name] = dict(mlups=r["mlups"], achieved_gbs=r["achieved_gbs"], kernel_ms=r["kernel_ms"],
                                      speedup_vs_fp32=r["mlups"] / fp32["mlups"], frac=r["achieved_gbs"] / HBM_PEAK_GBS)
     copy = copy_leg(torch)
     log(f"bench: device copy {copy['gbs']:.0f} GB/s")
+    pull = pull_pattern_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n)
+    log(f"bench: bare pull pattern (Stream) {pull['gbs']:.0f} GB/s ({pull['kernel_ms']:.4f} ms)")
     extras = {}
     if not args.no_extras:
         extras["periodic"] = periodic_leg(xa, torch, n, args.steps, args.warmup)
@@ -756,6 +797,8 @@ def main():
             "kernel_ms": fp32["kernel_ms"],
             "device_copy_gbs": copy["gbs"],
             "frac_of_device_copy": fp32["achieved_gbs"] / copy["gbs"],
+            "pull_pattern_gbs": pull["gbs"],
+            "frac_of_pull_pattern": fp32["achieved_gbs"] / pull["gbs"],
         },
         "cpu_baseline": cpu,
         "mlups_events": fp32["mlups_events"],

@@ -11,7 +11,11 @@
  * Field layout (identical to the reference's WarpGrid fields, xlb/grid/warp_grid.py:17-32):
  * structure of arrays, C order, shape (card, nx, ny, nz) with z unit-stride; 2-D lattices use
  * (card, nx, ny, 1).  `N = nx*ny*nz` is the number of cells.  Population l of cell
- * (x,y,z) lives at element  l*N + (x*ny + y)*nz + z.
+ * (x,y,z) lives at element  l*plane + (x*ny + y)*nz + z, where `plane` >= N is the field's plane
+ * stride in elements (0 = N, the dense layout).  Entry points that take population fields take
+ * one `plane` for all of them; other fields (rho, u, Pi, adjoint fields, masks) are dense.  A
+ * padded stride (xlbh_plane_stride) staggers the planes across HBM channels; the torch view
+ * (Q, nx, ny, nz) with strides (plane, ny*nz, nz, 1) keeps the reference's indexing.
  *
  * Grid limits: 1 <= nx <= 65535, N <= 2^32 - 256, and at most 65535 launch tiles of y rows
  * (ny / 4, ny / 2 or ny rows per tile for nz <= 64, <= 128 or more); entry points that build cell lists
@@ -46,8 +50,20 @@ enum {
   XLBH_F16 = 2,
   XLBH_BF16 = 3,
   XLBH_F16_SHIFTED = 4,
-  XLBH_BF16_SHIFTED = 5
+  XLBH_BF16_SHIFTED = 5,
+  XLBH_I16_BLOCK = 6
 };
+/* XLBH_I16_BLOCK (no counterpart in the reference): 16-bit block floating point.  Element l of a
+ * cell holds an int16 mantissa m of x = f_l / w_l - 1; one int32 exponent k per row segment gives
+ * f_l = w_l + m * (w_l * 2^-k) (rounded once to float).  A row segment is the run of bz cells
+ * along the fast axis (z in 3-D, y in 2-D) that one 256-thread sweep tile covers, bz = the
+ * smallest of 64, 128, 256 that is >= the fast extent (else 256).  The segments are numbered
+ * ((x * ny + y) * S + z / bz) in 3-D and (x * S + y / bz) in 2-D, S = ceil(fast extent / bz).
+ * The exponents (int32, one per segment) follow the last plane of the field, at byte offset
+ * ceil(Q * plane * 2 / 256) * 256 from its base; xlbh_population_bytes() gives the whole size.
+ * Each writing step chooses k per segment from the largest |x| it writes there (every
+ * direction, cells with id 255 excluded), so |m| <= 32767: the relative precision is 2^-15 of
+ * the segment's largest deviation instead of 2^-11 of each value (FP16). */
 
 /* Arithmetic precision of the per-cell update (PrecisionPolicy.compute_precision). */
 enum { XLBH_COMPUTE_F32 = 0, XLBH_COMPUTE_F64 = 1 };
@@ -93,6 +109,15 @@ const char* xlbh_last_error(void);
 /* ABI version of this header (incremented on incompatible change). */
 int xlbh_abi_version(void);
 
+/* Plane stride (elements) this library recommends for population fields of the given grid and
+ * store format: N plus a pad that staggers the Q plane starts across HBM channels (-1 for
+ * invalid arguments).  No counterpart in the reference (WarpGrid fields are dense). */
+int64_t xlbh_plane_stride(int lattice, int store, int nx, int ny, int nz);
+
+/* Bytes of one population field with plane stride `plane` (0 = N): Q * plane * S, and for
+ * XLBH_I16_BLOCK the segment exponents after it (-1 for invalid arguments). */
+int64_t xlbh_population_bytes(int lattice, int store, int nx, int ny, int nz, int64_t plane);
+
 /* ---------------------------------------------------------------------------------------
  * The fused step.  Replaces IncompressibleNavierStokesStepper.warp_implementation and its
  * kernel (xlb/operator/stepper/nse_stepper.py:245-392): pull streaming with periodic wrap,
@@ -121,7 +146,7 @@ int xlbh_step(int lattice, int collision, int store, int compute,
               const uint8_t* bc_mask, const uint32_t* missing_bits,
               const void* aux, const xlbh_bc_desc* bc_table,
               int force_mode, const double* force_uniform, const void* force_field,
-              int nx, int ny, int nz, double omega, double collision_param, int timestep, void* stream);
+              int nx, int ny, int nz, int64_t plane, double omega, double collision_param, int timestep, void* stream);
 
 /* n_steps fused steps with the buffer swap done in C (the caller's loop of
  * mlups_3d.py:78-80 moved below the ABI).  Step t reads f_a when t is even and f_b when t is
@@ -131,17 +156,17 @@ int xlbh_run(int lattice, int collision, int store, int compute,
              const uint8_t* bc_mask, const uint32_t* missing_bits,
              const void* aux, const xlbh_bc_desc* bc_table,
              int force_mode, const double* force_uniform, const void* force_field,
-             int nx, int ny, int nz, double omega, double collision_param, int timestep0, int n_steps,
+             int nx, int ny, int nz, int64_t plane, double omega, double collision_param, int timestep0, int n_steps,
              void* stream);
 
 /* n_states steps from f_start, keeping every state: state j (the output of step t0 + j) is
- * written to states + j * (Q N S_store) bytes, for j = 0 .. n_states - 1.  The forward pass of a
+ * written to states + j * xlbh_population_bytes(..., plane) bytes, for j = 0 .. n_states - 1.  The forward pass of a
  * checkpointed reverse sweep (xlbh_adjoint_run without XLBH_ADJ_RECOMPUTE); no counterpart in the
  * reference, whose tape keeps the segment's states itself. */
 int xlbh_record(int lattice, int collision, int store, int compute, const void* f_start, void* states,
                 const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
                 const xlbh_bc_desc* bc_table, int force_mode, const double* force_uniform,
-                const void* force_field, int nx, int ny, int nz, double omega, double collision_param,
+                const void* force_field, int nx, int ny, int nz, int64_t plane, double omega, double collision_param,
                 int timestep0,
                 int n_states, void* stream);
 
@@ -151,19 +176,19 @@ int xlbh_record(int lattice, int collision, int store, int compute, const void* 
 
 /* Pull streaming with periodic wrap: f_out[l](x) = f_in[l](x - c_l).  stream.py:59-99. */
 int xlbh_stream(int lattice, int store, const void* f_in, void* f_out,
-                int nx, int ny, int nz, void* stream);
+                int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* f = feq(rho, u).  rho: (1, N), u: (D, N) in field type `field`.  quadratic_equilibrium.py:35-98. */
 int xlbh_equilibrium(int lattice, int store, int compute, int field,
                      const void* rho, const void* u, void* f,
-                     int nx, int ny, int nz, void* stream);
+                     int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* Moments of f.  which: 0 = rho and u (macroscopic.py:28-66), 1 = rho only
  * (zero_moment.py:22-47), 2 = u given rho (first_moment.py:25-66, rho is an input),
  * 3 = second moment Pi (second_moment.py:68-101, written to `pi` with D(D+1)/2 rows). */
 int xlbh_moments(int lattice, int store, int compute, int field, int which,
                  const void* f, void* rho, void* u, void* pi,
-                 int nx, int ny, int nz, void* stream);
+                 int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* fout = collision(f, feq, rho, u, omega) (+ exact-difference force).  bgk.py:30-81,
  * kbc.py:268-347, forced_collision.py:41-104.  rho, u: (1,N), (D,N) in type `field`. */
@@ -171,7 +196,7 @@ int xlbh_collide(int lattice, int collision, int store, int compute, int field,
                  const void* f, const void* feq, void* fout,
                  const void* rho, const void* u, double omega, double collision_param,
                  int force_mode, const double* force_uniform, const void* force_field,
-                 int nx, int ny, int nz, void* stream);
+                 int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* Standalone boundary-condition application (BoundaryCondition._construct_kernel,
  * boundary_condition.py:86-117): at cells with bc_mask == bc_id, f_post = functional(
@@ -181,7 +206,7 @@ int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id,
                   const xlbh_bc_desc* desc,
                   const void* f_pre, void* f_post,
                   const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
-                  int nx, int ny, int nz, void* stream);
+                  int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Boundary masks (integer; bit-exact).
@@ -228,12 +253,12 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
                            const xlbh_bc_desc* desc,
                            const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits,
                            const void* aux, double* out, void* workspace,
-                           int nx, int ny, int nz, void* stream);
+                           int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* Global mass and momentum: out[0] = sum rho, out[1..3] = sum rho*u (= sum c f) over every
  * cell whose bc_mask is not 255 (bc_mask may be NULL). */
 int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask,
-                       double* out, void* workspace, int nx, int ny, int nz, void* stream);
+                       double* out, void* workspace, int nx, int ny, int nz, int64_t plane, void* stream);
 
 /* Surface voxelisation of a triangle mesh into the masks, replacing the reference's
  * MeshBoundaryMasker (xlb/operator/boundary_masker/mesh_boundary_masker.py:49-236).
@@ -291,14 +316,15 @@ int xlbh_adjoint_step(int lattice, int collision, int store, int compute,
                       const void* aux, const xlbh_bc_desc* bc_table,
                       const int32_t* bc_cells, int n_bc_cells,
                       int force_mode, const double* force_uniform, const void* force_field,
-                      double* grad_force, int nx, int ny, int nz, double omega, double collision_param,
+                      double* grad_force, int nx, int ny, int nz, int64_t plane, double omega, double collision_param,
                       void* stream);
 
 /* Reverse sweep over one checkpoint segment of n_steps steps, below the ABI (the counterpart
  * of xlbh_run for the backward pass; the reference replays a Warp tape per segment,
  * autodiff_stepper_subroutine.py:459-466).
  *   f_ckpt : the state f^{t0} at the segment start (store format), read only.
- *   states : workspace of (n_steps - 1) fields (Q, N) in the store format; receives the
+ *   states : workspace of (n_steps - 1) population fields of xlbh_population_bytes(..., plane)
+ *            bytes each, in the store format; receives the
  *            recomputed f^{t0+1} .. f^{t0+n_steps-1}.  May be NULL when n_steps == 1.
  *   lam_a  : dL/df^{t0+n_steps} on entry.  lam_b: a second (Q, N) compute-type buffer.
  * The result dL/df^{t0} is left in lam_a when n_steps is even and in lam_b when it is odd; the
@@ -318,7 +344,7 @@ int xlbh_adjoint_run(int lattice, int collision, int store, int compute,
                      const void* aux, const xlbh_bc_desc* bc_table,
                      const int32_t* bc_cells, int n_bc_cells,
                      int force_mode, const double* force_uniform, const void* force_field,
-                     double* grad_force, int nx, int ny, int nz, double omega, double collision_param,
+                     double* grad_force, int nx, int ny, int nz, int64_t plane, double omega, double collision_param,
                      int timestep0, int n_steps, void* workspace, int flags, void* stream);
 
 #define XLBH_ADJ_RECOMPUTE 1

@@ -4,7 +4,9 @@ TEST INFRASTRUCTURE ONLY: the parity checker of the HIP backend and the CPU base
 bench.py.  Nothing in the product package imports this module.
 
 The oracle takes plain NumPy arrays in the reference's layout: populations (q, nx, ny, nz) in the
-store format (float32 / float64 / uint16 bit patterns for FP16 and BF16), bc_mask (1, nx, ny, nz)
+store format (float32 / float64 / uint16 bit patterns for FP16 and BF16; int16 mantissas for the
+block format, as the view at the start of a buffer that holds the segment exponents after the
+planes, like the GPU fields), bc_mask (1, nx, ny, nz)
 uint8, missing mask (q, nx, ny, nz) bool, and a list of BC descriptions
 ``dict(id=, kind=, rho=, u=)`` in the stepper's list order.
 """
@@ -24,7 +26,7 @@ ZOUHE_VELOCITY, ZOUHE_PRESSURE, REGULARIZED_VELOCITY, REGULARIZED_PRESSURE = 5, 
 EXTRAPOLATION_OUTFLOW, GRADS_APPROXIMATION = 9, 10
 
 # store formats
-F32, F64, F16, BF16, F16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3, 4, 5
+F32, F64, F16, BF16, F16_SHIFTED, BF16_SHIFTED, I16_BLOCK = 0, 1, 2, 3, 4, 5, 6
 
 
 class OrcBc(ctypes.Structure):
@@ -45,7 +47,7 @@ def lib():
             build()
         L = ctypes.CDLL(LIB_PATH)
         vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
-        L.orc_step.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, i, i, vp, vp, i, i, i, d, d]
+        L.orc_step.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, i, i, vp, vp, i, i, i, d, d, vp]
         L.orc_aux_init.argtypes = [i, vp, vp, i, vp, i, i, i]
         L.orc_equilibrium.argtypes = [i, i, i, vp, vp, vp, i, i, i]
         L.orc_macroscopic.argtypes = [i, i, i, vp, vp, vp, i, i, i]
@@ -122,9 +124,11 @@ def lattice(lat):
     return dict(c=c[:, :q], w=w[:q], opp=opp[:q], qi=qi[:q])
 
 
-def step(lat, coll, store, compute, f0, f1, bc_mask, missing, bcs, omega, force=None, force_field=None, cparam=0.0):
+def step(lat, coll, store, compute, f0, f1, bc_mask, missing, bcs, omega, force=None, force_field=None, cparam=0.0,
+         aux=None):
     """One fused step in place: reads f0 (receiving aux recovery writes), writes f1.  coll: 0 BGK,
-    1 KBC, 2 Smagorinsky BGK with coefficient cparam."""
+    1 KBC, 2 Smagorinsky BGK with coefficient cparam.  aux: (N,) prescribed values for the block
+    format (I16_BLOCK), whose int16 mantissas cannot carry the reference's f_1[0] stash."""
     nx, ny, nz = _dims(f0)
     bm = None if bc_mask is None else np.ascontiguousarray(bc_mask, dtype=np.uint8)
     mm = None if missing is None else np.ascontiguousarray(missing, dtype=np.uint8)
@@ -136,8 +140,9 @@ def step(lat, coll, store, compute, f0, f1, bc_mask, missing, bcs, omega, force=
         mode, fu = 1, np.ascontiguousarray(list(force) + [0.0] * (3 - len(force)), dtype=np.float64)
     if bm is None:
         bm = np.zeros((1, nx, ny, nz), dtype=np.uint8)
+    ax = None if aux is None else np.ascontiguousarray(np.broadcast_to(aux, (nx * ny * nz,)), dtype=np.float64)
     rc = lib().orc_step(lat, coll, store, compute, _p(f0), _p(f1), _p(bm), _p(mm), arr, len(bcs), mode, _p(fu), _p(ff),
-                        nx, ny, nz, float(omega), float(cparam))
+                        nx, ny, nz, float(omega), float(cparam), _p(ax))
     assert rc == 0
 
 

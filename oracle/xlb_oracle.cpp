@@ -208,8 +208,104 @@ template <>
 struct Store<4> : Store<2> {};
 template <>
 struct Store<5> : Store<3> {};
+// Format 6, 16-bit block floating point: int16 mantissas whose values need the segment exponent
+// (Ctx::rd / the step's segment quantisation below); get/put are never used for it.
+template <>
+struct Store<6> {
+  using T = int16_t;
+  static double get(T) { return 0.0; }
+  static T put(double) { return 0; }
+};
 
 constexpr bool shifted(int fmt) { return fmt == 4 || fmt == 5; }
+constexpr bool block(int fmt) { return fmt == 6; }
+
+// ------------------------------------------------------------------------------------------
+// Block floating point (format 6; no counterpart in the reference).  Element l of a cell holds
+// m = rint_even(t), t = float(2^k (f_l / w_l - 1)) formed as one fused multiply-add, and every row
+// segment (the bz cells along the fast axis that one 256-thread tile of the GPU sweep covers)
+// shares one exponent k, chosen from the largest |x| = |fma(f_l, 1/w_l, -1)| the step writes
+// there: M' = float(M * (1 + 2^-15)), k = 14 - (unbiased exponent of M'), clamped to [-64, 60];
+// a segment whose M is not finite gets k = -127 and m = 32767 everywhere (decodes to +inf).
+// f = fma(m, w 2^-k, w).  w is the float-rounded lattice weight; 1/w is exact in float.  The
+// exponents follow the Q planes, at byte offset ceil(Q N 2 / 256) 256.
+// ------------------------------------------------------------------------------------------
+constexpr int kBlkKMin = -64, kBlkKMax = 60, kBlkNonFinite = -127;
+
+struct BlkGeom {
+  int nx = 1, ny = 1, nz = 1, lbz = 6, S = 1;
+  size_t seg(int x, int y, int z) const {
+    if (nz > 1) return ((size_t)x * ny + y) * S + (size_t)(z >> lbz);
+    return (size_t)x * S + (size_t)(y >> lbz);
+  }
+  size_t count() const { return (nz > 1 ? (size_t)nx * ny : (size_t)nx) * S; }
+};
+
+BlkGeom blk_geom(int nx, int ny, int nz) {
+  BlkGeom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  const int fast = nz > 1 ? nz : ny;
+  g.lbz = 6;
+  while (g.lbz < 8 && (1 << g.lbz) < fast) ++g.lbz;
+  g.S = (fast + (1 << g.lbz) - 1) >> g.lbz;
+  return g;
+}
+
+size_t blk_offset(int q, size_t N) { return ((size_t)q * N * 2 + 255) / 256 * 256; }
+
+int blk_k_from_max(float m) {
+  if (!(m <= 3.0e38f)) return kBlkNonFinite;
+  const float mp = m * 1.000030517578125f;
+  const int e = (int)((fbits(mp) >> 23) & 0xffu) - 127;
+  const int k = 14 - e;
+  return k < kBlkKMin ? kBlkKMin : (k > kBlkKMax ? kBlkKMax : k);
+}
+
+float blk_decode(int16_t m, float w, int k) {
+  return std::fmaf((float)m, bitsf(fbits(w) - ((uint32_t)k << 23)), w);
+}
+
+int16_t blk_encode(float v, float inv_w, int k) {
+  if (k == kBlkNonFinite) return 32767;
+  const float t = std::fmaf(v, bitsf(fbits(inv_w) + ((uint32_t)k << 23)), -bitsf((uint32_t)(k + 127) << 23));
+  const float y = t + 12582912.0f;  // 1.5 * 2^23: the float sum rounds t to an integer (ties to even)
+  return (int16_t)(fbits(y) & 0xffffu);
+}
+
+// Quantise the float values v (q, N) of the written cells into the block field f (mantissas, then
+// the exponents at blk_offset), segment by segment.
+void blk_store(const double* w, int q, const BlkGeom& g, const float* v, const uint8_t* written, int16_t* f) {
+  const size_t N = (size_t)g.nx * g.ny * g.nz;
+  int32_t* e = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(f) + blk_offset(q, N));
+  const int bz = 1 << g.lbz, fast = g.nz > 1 ? g.nz : g.ny;
+  const int rows = g.nz > 1 ? g.nx * g.ny : g.nx;
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < rows; ++r)
+    for (int sb = 0; sb < g.S; ++sb) {
+      const int f0 = sb * bz, f1 = std::min(fast, f0 + bz);
+      // cells of the segment: consecutive along the fast axis (z in 3-D, y in 2-D: also unit stride)
+      const size_t base = (size_t)r * fast;
+      float M = 0.0f;
+      bool nan = false;
+      for (int i = f0; i < f1; ++i) {
+        if (!written[base + i]) continue;
+        for (int l = 0; l < q; ++l) {
+          const float x = std::fabs(std::fmaf(v[(size_t)l * N + base + i], (float)(1.0 / w[l]), -1.0f));
+          if (x != x) nan = true;
+          else if (x > M) M = x;
+        }
+      }
+      const int k = nan ? kBlkNonFinite : blk_k_from_max(M);
+      e[(size_t)r * g.S + sb] = k;
+      for (int i = f0; i < f1; ++i) {
+        if (!written[base + i]) continue;
+        for (int l = 0; l < q; ++l)
+          f[(size_t)l * N + base + i] = blk_encode(v[(size_t)l * N + base + i], (float)(1.0 / w[l]), k);
+      }
+    }
+}
 
 // ------------------------------------------------------------------------------------------
 // Boundary descriptions passed from Python (one per BC instance, in the stepper's list order).
@@ -243,13 +339,21 @@ struct Ctx {
   size_t N;
   C omega;
   C cparam = 0;  // Smagorinsky coefficient
+  BlkGeom g;     // block format: segment geometry
   int force_mode;
   C force_u[3];
   const double* force_field;  // (d, N)
+  const double* aux_side = nullptr;  // block format: prescribed values per cell (no f_1[0] stash)
 
   size_t cell(int x, int y, int z) const { return ((size_t)x * ny + y) * nz + z; }
-  // compute-type read of a population (shifted formats add w_l back)
+  // compute-type read of a population (shifted formats add w_l back; the block format decodes
+  // with the cell's segment exponent, stored after the field's planes)
   C rd(const S* f, int l, size_t c) const {
+    if constexpr (block(FMT)) {
+      const int32_t* e = reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(f) + blk_offset(L->q, N));
+      const int z = (int)(c % nz), y = (int)((c / nz) % ny), x = (int)(c / ((size_t)nz * ny));
+      return (C)blk_decode(f[(size_t)l * N + c], (float)L->w[l], e[g.seg(x, y, z)]);
+    }
     C v = (C)Store<FMT>::get(f[(size_t)l * N + c]);
     if (shifted(FMT)) v += (C)L->w[l];
     return v;
@@ -492,7 +596,7 @@ void bc_functional(const Ctx<FMT, C>& X, const OrcBc& bc, int x, int y, int z, c
       C n[3];
       get_normal(L, mm, n);
       const C fsum = get_bc_fsum(L, out, mm);
-      const C pv = X.rd_raw(f1, 0, c);
+      const C pv = block(FMT) ? (C)X.aux_side[c] : X.rd_raw(f1, 0, c);
       C u[3] = {0, 0, 0};
       for (int d = 0; d < L.d; ++d) u[d] = -pv * n[d];
       C un = 0;
@@ -508,7 +612,7 @@ void bc_functional(const Ctx<FMT, C>& X, const OrcBc& bc, int x, int y, int z, c
     case 8: {  // pressure: ZouHe (bc_zouhe.py:314-338), Regularized (bc_regularized.py:170-196)
       C n[3];
       get_normal(L, mm, n);
-      const C rho = X.rd_raw(f1, 0, c);
+      const C rho = block(FMT) ? (C)X.aux_side[c] : X.rd_raw(f1, 0, c);
       const C fsum = get_bc_fsum(L, out, mm);
       const C un = -(C)1.0 + fsum / rho;
       C u[3] = {0, 0, 0};
@@ -592,7 +696,7 @@ void step_grid(Ctx<FMT, C>& X, int coll) {
   using S = typename Ctx<FMT, C>::S;
   // Snapshot f_1 at aux-recovery cells before it is overwritten (nse_stepper.py:310, 318-342).
   std::vector<size_t> aux_cells;
-  for (size_t c = 0; c < X.N; ++c) {
+  for (size_t c = 0; c < (block(FMT) ? 0 : X.N); ++c) {
     const int id = X.bc_mask[c];
     for (int i = 0; i < X.nbc; ++i)
       if (needs_aux_recovery(X.bcs[i].kind) && X.bcs[i].id == id) {
@@ -603,6 +707,13 @@ void step_grid(Ctx<FMT, C>& X, int coll) {
   std::vector<S> f1_snap(aux_cells.size() * Q);
   for (size_t k = 0; k < aux_cells.size(); ++k)
     for (int l = 0; l < Q; ++l) f1_snap[k * Q + l] = X.f1[(size_t)l * X.N + aux_cells[k]];
+  // block format: the step's float outputs, quantised segment by segment once all are known
+  std::vector<float> blk_out;
+  std::vector<uint8_t> blk_written;
+  if constexpr (block(FMT)) {
+    blk_out.assign((size_t)Q * X.N, 0.0f);
+    blk_written.assign(X.N, 0);
+  }
 
 #pragma omp parallel for schedule(static)
   for (int x = 0; x < X.nx; ++x) {
@@ -632,9 +743,15 @@ void step_grid(Ctx<FMT, C>& X, int coll) {
           exact_difference(L, fpc, feq, rho, u, F);
         }
         apply_bc(X, x, y, z, id, mm, X.f1, fps, fpc, false, tmp);
-        for (int l = 0; l < Q; ++l) X.wr(X.f1, l, c, tmp[l]);
+        if constexpr (block(FMT)) {
+          for (int l = 0; l < Q; ++l) blk_out[(size_t)l * X.N + c] = (float)tmp[l];
+          blk_written[c] = 1;
+        } else {
+          for (int l = 0; l < Q; ++l) X.wr(X.f1, l, c, tmp[l]);
+        }
       }
   }
+  if constexpr (block(FMT)) blk_store(L.w, Q, X.g, blk_out.data(), blk_written.data(), X.f1);
   // Aux recovery into f_0 (after every read of f_0 by this step).
   for (size_t k = 0; k < aux_cells.size(); ++k) {
     const size_t c = aux_cells[k];
@@ -657,6 +774,7 @@ int dispatch(int store, int compute, Fn&& fn) {
       case 3: return fn(std::integral_constant<int, 3>{}, float{});
       case 4: return fn(std::integral_constant<int, 4>{}, float{});
       case 5: return fn(std::integral_constant<int, 5>{}, float{});
+      case 6: return fn(std::integral_constant<int, 6>{}, float{});
     }
   } else {
     switch (store) {
@@ -703,9 +821,11 @@ int orc_set_threads(int n) {
 }
 
 // One fused step.  f0 is read (and receives aux recovery writes), f1 is written.
+// aux: the block format's prescribed values per cell (N doubles), in place of the f_1[0] stash an
+// int16 mantissa cannot hold; NULL for every other format.
 int orc_step(int lat, int coll, int store, int compute, void* f0, void* f1, const uint8_t* bc_mask,
              const uint8_t* missing, const OrcBc* bcs, int nbc, int force_mode, const double* force_u,
-             const double* force_field, int nx, int ny, int nz, double omega, double cparam) {
+             const double* force_field, int nx, int ny, int nz, double omega, double cparam, const double* aux) {
   return dispatch(store, compute, [&](auto S, auto c) -> int {
     constexpr int FMT = decltype(S)::value;
     using C = decltype(c);
@@ -726,6 +846,12 @@ int orc_step(int lat, int coll, int store, int compute, void* f0, void* f1, cons
     X.force_mode = force_mode;
     for (int d = 0; d < 3; ++d) X.force_u[d] = force_u ? (C)force_u[d] : (C)0;
     X.force_field = force_field;
+    X.g = blk_geom(nx, ny, nz);
+    if constexpr (block(FMT)) {
+      X.aux_side = aux;
+      for (int i = 0; i < nbc; ++i)
+        if (needs_aux_recovery(bcs[i].kind) && !aux) return 3;  // prescribed values need the side table
+    }
     step_grid(X, coll);
     return 0;
   });
@@ -737,6 +863,7 @@ int orc_aux_init(int store, void* f1, const uint8_t* bc_mask, int id, const doub
   const size_t N = (size_t)nx * ny * nz;
   return dispatch(store, 0, [&](auto S, auto) -> int {
     constexpr int FMT = decltype(S)::value;
+    if constexpr (block(FMT)) return 3;
     auto* f = static_cast<typename Store<FMT>::T*>(f1);
     for (size_t c = 0; c < N; ++c)
       if (bc_mask[c] == id) f[c] = Store<FMT>::put(values[c]);
@@ -755,11 +882,21 @@ int orc_equilibrium(int lat, int store, int compute, const double* rho, const do
     X.N = (size_t)nx * ny * nz;
     const Lattice& L = *X.L;
     auto* ff = static_cast<typename Ctx<FMT, C>::S*>(f);
+    std::vector<float> out(block(FMT) ? (size_t)L.q * X.N : 0);
     for (size_t cc = 0; cc < X.N; ++cc) {
       C u3[3] = {0, 0, 0}, feq[27];
       for (int d = 0; d < L.d; ++d) u3[d] = (C)u[(size_t)d * X.N + cc];
       equilibrium(L, (C)rho[cc], u3, feq);
-      for (int l = 0; l < L.q; ++l) X.wr(ff, l, cc, feq[l]);
+      for (int l = 0; l < L.q; ++l) {
+        if constexpr (block(FMT))
+          out[(size_t)l * X.N + cc] = (float)feq[l];
+        else
+          X.wr(ff, l, cc, feq[l]);
+      }
+    }
+    if constexpr (block(FMT)) {
+      const std::vector<uint8_t> all(X.N, 1);
+      blk_store(L.w, L.q, blk_geom(nx, ny, nz), out.data(), all.data(), ff);
     }
     return 0;
   });
@@ -774,6 +911,10 @@ int orc_macroscopic(int lat, int store, int compute, const void* f, double* rho,
     X.L = &g_lat[lat];
     X.N = (size_t)nx * ny * nz;
     const Lattice& L = *X.L;
+    X.nx = nx;
+    X.ny = ny;
+    X.nz = nz;
+    X.g = blk_geom(nx, ny, nz);
     auto* ff = static_cast<const typename Ctx<FMT, C>::S*>(f);
     for (size_t cc = 0; cc < X.N; ++cc) {
       C fl[27], r, uu[3];
@@ -790,6 +931,7 @@ int orc_macroscopic(int lat, int store, int compute, const void* f, double* rho,
 int orc_stream(int lat, int store, const void* fin, void* fout, int nx, int ny, int nz) {
   return dispatch(store, 0, [&](auto S, auto) -> int {
     constexpr int FMT = decltype(S)::value;
+    if constexpr (block(FMT)) return 3;  // the block format is written by the step and equilibrium only
     using T = typename Store<FMT>::T;
     const Lattice& L = g_lat[lat];
     const T* a = static_cast<const T*>(fin);
@@ -821,6 +963,7 @@ int orc_collide(int lat, int coll, int store, int compute, const void* f, const 
                 const double* force_u, const double* force_field, int nx, int ny, int nz) {
   return dispatch(store, compute, [&](auto S, auto c) -> int {
     constexpr int FMT = decltype(S)::value;
+    if constexpr (block(FMT)) return 3;  // the block format is written by the step and equilibrium only
     using C = decltype(c);
     Ctx<FMT, C> X;
     X.L = &g_lat[lat];
@@ -855,6 +998,7 @@ int orc_apply_bc(int lat, int store, int compute, const OrcBc* bc, const void* f
                  const uint8_t* bc_mask, const uint8_t* missing, const double* aux, int nx, int ny, int nz) {
   return dispatch(store, compute, [&](auto S, auto c) -> int {
     constexpr int FMT = decltype(S)::value;
+    if constexpr (block(FMT)) return 3;  // the block format is written by the step and equilibrium only
     using C = decltype(c);
     using T = typename Ctx<FMT, C>::S;
     Ctx<FMT, C> X;
@@ -1036,6 +1180,7 @@ int orc_momentum_transfer(int lat, int store, int compute, const OrcBc* bc, cons
     X.ny = ny;
     X.nz = nz;
     X.N = (size_t)nx * ny * nz;
+    X.g = blk_geom(nx, ny, nz);
     const Lattice& L = *X.L;
     std::vector<T> aux_field(X.N);
     for (size_t cc = 0; cc < X.N; ++cc) aux_field[cc] = Store<FMT>::put(aux ? aux[cc] : 0.0);

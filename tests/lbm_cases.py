@@ -19,7 +19,53 @@ STORE_NP = {
     orc.BF16: np.uint16,
     orc.F16_SHIFTED: np.uint16,
     orc.BF16_SHIFTED: np.uint16,
+    orc.I16_BLOCK: np.int16,
 }
+
+
+# ---- block format (int16 mantissas + one int32 exponent per row segment after the planes) ----
+def block_geometry(shape3):
+    """(bz, segments per row, rows) of the block format for grid (nx, ny, nz)."""
+    nx, ny, nz = shape3
+    fast = nz if nz > 1 else ny
+    bz = next((b for b in (64, 128, 256) if b >= fast), 256)
+    return bz, -(-fast // bz), (nx * ny if nz > 1 else nx)
+
+
+def block_layout(q, shape3):
+    """(byte offset of the exponents, number of segments) of a dense block field."""
+    n = int(np.prod(shape3))
+    _, segs, rows = block_geometry(shape3)
+    return -(-(q * n * 2) // 256) * 256, segs * rows
+
+
+def pop_alloc(fmt, q, shape3):
+    """Host population field in store format fmt; a block field is the (q, ...) view at the start
+    of a buffer that also holds its exponents."""
+    if fmt != orc.I16_BLOCK:
+        return np.zeros((q,) + tuple(shape3), dtype=STORE_NP[fmt])
+    off, nseg = block_layout(q, shape3)
+    buf = np.zeros((off + 4 * nseg) // 2, dtype=np.int16)
+    return buf[: q * int(np.prod(shape3))].reshape((q,) + tuple(shape3))
+
+
+def _block_buffer(a):
+    b = a
+    while b.base is not None and isinstance(b.base, np.ndarray):
+        b = b.base
+    return b
+
+
+def block_exps(a):
+    """The int32 segment exponents of a host block field (a view into its buffer)."""
+    off, nseg = block_layout(a.shape[0], a.shape[1:])
+    return _block_buffer(a)[off // 2: off // 2 + 2 * nseg].view(np.int32)
+
+
+def pop_copy(a):
+    if a.dtype != np.int16:
+        return a.copy()
+    return _block_buffer(a).copy()[: a.size].reshape(a.shape)
 
 
 def policy_codes(policy):
@@ -27,9 +73,16 @@ def policy_codes(policy):
 
 
 def to_np(t):
-    """GPU tensor -> numpy array holding the stored bit pattern."""
+    """GPU tensor -> numpy array holding the stored bit pattern (a block field with its exponents)."""
     import torch
 
+    if t.dtype == torch.int16:
+        import xlb_amd as xa
+
+        a = pop_alloc(orc.I16_BLOCK, t.shape[0], tuple(t.shape[1:]))
+        a[...] = t.detach().cpu().numpy()
+        block_exps(a)[...] = xa.grid.block_exponents(t).cpu().numpy()
+        return a
     if t.dtype in (torch.float16, torch.bfloat16):
         return t.detach().view(torch.int16).cpu().numpy().view(np.uint16)
     if t.dtype == torch.bool:
@@ -46,8 +99,33 @@ def from_np(a, like):
     return torch.from_numpy(np.ascontiguousarray(a)).to(like.device)
 
 
+def upload(a, t):
+    """Host stored populations (with a block field's exponents) -> the GPU field t, in place."""
+    import torch
+
+    if t.dtype == torch.int16:
+        import xlb_amd as xa
+
+        t.copy_(torch.from_numpy(np.ascontiguousarray(a)).to(t.device))
+        xa.grid.block_exponents(t).copy_(torch.from_numpy(block_exps(a).copy()).to(t.device))
+        return t
+    return t.copy_(from_np(a, t))
+
+
 def decode(a, fmt, w):
-    """Stored populations -> float64 values of f."""
+    """Stored populations -> float64 values of f (block: w (1 + m 2^-k) with w rounded to float,
+    the value the kernels decode to before their single rounding)."""
+    if fmt == orc.I16_BLOCK:
+        q, shape3 = a.shape[0], a.shape[1:]
+        bz, segs, _ = block_geometry(shape3)
+        k = block_exps(a).astype(np.float64)
+        nx, ny, nz = shape3
+        if nz > 1:
+            k = np.repeat(k.reshape(nx, ny, segs), bz, axis=2)[:, :, :nz]
+        else:
+            k = np.repeat(k.reshape(nx, segs), bz, axis=1)[:, :ny, None]
+        wf = np.asarray(w, dtype=np.float32).astype(np.float64).reshape((-1, 1, 1, 1))
+        return wf * (1.0 + a.astype(np.float64) * np.exp2(-k)[None])
     if fmt in (orc.F32, orc.F64):
         v = a.astype(np.float64)
     elif fmt in (orc.F16, orc.F16_SHIFTED):
@@ -129,7 +207,7 @@ def oracle_aux(bc, bc_mask, d, policy):
         if v.size == 1:
             v = np.full(cells.size, v[0])
         t = torch.from_numpy(v)
-        if not policy.shifted:
+        if not (policy.shifted or policy.block):
             t = t.to(policy.store_precision.torch_dtype)
         vals[cells] = t.to(policy.compute_precision.torch_dtype).double().numpy()
     return vals
@@ -278,11 +356,18 @@ LARGE_CASES = {
     # the benchmark workload (mlups_3d.py:45-90) at edge 256, and at 64
     "mlups_cavity256": ("D3Q19", (256, 256, 256), "BGK", 1.0, "rest"),
     "mlups_cavity64": ("D3Q19", (64, 64, 64), "BGK", 1.0, "rest"),
+    # rows longer than one 256-cell tile (several block-format segments per row)
+    "long_periodic3d": ("D3Q19", (4, 6, 600), "BGK", 1.7, "random"),
+    "long_periodic2d_kbc": ("D2Q9", (6, 520), "KBC", 1.6, "random"),
+    "long_cavity3d": ("D3Q19", (6, 5, 300), "BGK", 1.2, "rest"),
+    "long_cavity2d": ("D2Q9", (5, 300), "BGK", 1.5, "rest"),
 }
 BC_SET = {"mlups_cavity256": "cavity_fullway", "mlups_cavity64": "cavity_fullway", "sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "c3_tgv": "periodic", "cavity_smagorinsky": "cavity_halfway",
           "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d",
           "sphere3d_grads_bgk": "sphere3d_grads", "tiny_periodic3d": "periodic", "tiny_periodic2d": "periodic",
-          "tiny_periodic_kbc": "periodic", "tiny_cavity2d": "cavity2d", "tiny_cavity3d": "cavity_fullway"}
+          "tiny_periodic_kbc": "periodic", "tiny_cavity2d": "cavity2d", "tiny_cavity3d": "cavity_fullway",
+          "long_periodic3d": "periodic", "long_periodic2d_kbc": "periodic", "long_cavity3d": "cavity_halfway",
+          "long_cavity2d": "cavity2d"}
 
 ORACLE_COLLISION = {"BGK": 0, "KBC": 1, "SmagorinskyLESBGK": 2}
 
@@ -332,7 +417,7 @@ class Pair:
         self.shape3 = shape3
         # oracle state
         self.o_mask, self.o_missing = oracle_masks(self.lat, vs.q, vs.d, shape3, self.bcs)
-        f_init = np.zeros((vs.q,) + shape3, dtype=STORE_NP[self.fmt])
+        f_init = pop_alloc(self.fmt, vs.q, shape3)
         if init == "tgv":
             g = [2 * np.pi / s * np.arange(s) for s in shape3]
             X, Y, Z = np.meshgrid(*g, indexing="ij", sparse=True)
@@ -341,23 +426,36 @@ class Pair:
             u[1] = -0.04 * np.cos(X) * np.sin(Y) * np.cos(Z)
             rho = np.ones(shape3)
             orc.equilibrium(self.lat, self.fmt, self.comp, rho, u, f_init)
-            self.f0.copy_(from_np(f_init, self.f0))
-            self.f1.copy_(self.f0)
+            upload(f_init, self.f0)
+            xa.grid.copy_population(self.f1, self.f0)
         elif init == "random":
             rho, u = random_init(vs, shape3, seed)
             orc.equilibrium(self.lat, self.fmt, self.comp, rho, u, f_init)
-            self.f0.copy_(from_np(f_init, self.f0))
-            self.f1.copy_(self.f0)
+            upload(f_init, self.f0)
+            xa.grid.copy_population(self.f1, self.f0)
         else:
             rho, u = np.ones(shape3), np.zeros((vs.d,) + shape3)
             orc.equilibrium(self.lat, self.fmt, self.comp, rho, u, f_init)
         self.init_rho, self.init_u = rho, u
-        self.o_f0 = f_init.copy()
-        self.o_f1 = f_init.copy()
+        self.o_f0 = pop_copy(f_init)
+        self.o_f1 = pop_copy(f_init)
+        self.o_aux = self._oracle_aux(policy)
+        self.o_bcs = [oracle_bc_desc(bc) for bc in self.bcs]
+
+    def _oracle_aux(self, policy, f1=None):
+        """Prescribed values: stashed in f_1[0] as the reference does, or, for the block format
+        (whose int16 mantissas cannot hold them), returned as the oracle's side table."""
+        fmt = policy_codes(policy)[0]
+        f1 = self.o_f1 if f1 is None else f1
+        side = np.zeros(self.o_mask.size)
         for bc in self.bcs:
             if bc.needs_aux_init:
-                orc.aux_init(self.fmt, self.o_f1, self.o_mask, bc.id, oracle_aux(bc, self.o_mask, vs.d, policy))
-        self.o_bcs = [oracle_bc_desc(bc) for bc in self.bcs]
+                vals = oracle_aux(bc, self.o_mask, self.vs.d, policy)
+                if fmt == orc.I16_BLOCK:
+                    side[self.o_mask.reshape(-1) == bc.id] = vals[self.o_mask.reshape(-1) == bc.id]
+                else:
+                    orc.aux_init(fmt, f1, self.o_mask, bc.id, vals)
+        return side if fmt == orc.I16_BLOCK else None
 
     def step_product(self, n, use_run=False):
         if use_run:
@@ -370,22 +468,21 @@ class Pair:
     def step_oracle(self, n):
         for _ in range(n):
             orc.step(self.lat, self.o_coll, self.fmt, self.comp, self.o_f0, self.o_f1, self.o_mask, self.o_missing,
-                     self.o_bcs, self.omega, force=self.force, force_field=self.force_field_np, cparam=self.cparam)
+                     self.o_bcs, self.omega, force=self.force, force_field=self.force_field_np, cparam=self.cparam,
+                     aux=self.o_aux)
             self.o_f0, self.o_f1 = self.o_f1, self.o_f0
 
     def oracle_reference_f(self, steps, policy):
         """Decoded populations of an oracle-only run of the same case under another policy
         (e.g. FP64FP64: the noise floor against which FP32 results are judged)."""
         fmt, comp = policy_codes(policy)
-        f0 = np.zeros((self.vs.q,) + self.shape3, dtype=STORE_NP[fmt])
+        f0 = pop_alloc(fmt, self.vs.q, self.shape3)
         orc.equilibrium(self.lat, fmt, comp, self.init_rho, self.init_u, f0)
-        f1 = f0.copy()
-        for bc in self.bcs:
-            if bc.needs_aux_init:
-                orc.aux_init(fmt, f1, self.o_mask, bc.id, oracle_aux(bc, self.o_mask, self.vs.d, policy))
+        f1 = pop_copy(f0)
+        aux = self._oracle_aux(policy, f1)
         for _ in range(steps):
             orc.step(self.lat, self.o_coll, fmt, comp, f0, f1, self.o_mask, self.o_missing, self.o_bcs, self.omega,
-                     force=self.force, force_field=self.force_field_np, cparam=self.cparam)
+                     force=self.force, force_field=self.force_field_np, cparam=self.cparam, aux=aux)
             f0, f1 = f1, f0
         return decode(f0, fmt, self.vs.w)
 

@@ -266,17 +266,17 @@ def test_adjoint_rejects_bad_arguments():
     from xlb_amd import _lib
     c3 = Case("periodic3d_uniform", P.FP64FP64)
     lam3 = torch.zeros_like(c3.f0)
-    nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = c3.st._launch_args(c3.f0, None, c3.bc_mask, c3.missing)
+    nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, ff = c3.st._launch_args(c3.f0, None, c3.bc_mask, c3.missing)
     rc = _lib.lib().xlbh_adjoint_step(c3.st.lattice_id, _lib.KBC, c3.st.precision_policy.store_format,
                                       c3.st.compute_id, _lib.ptr(c3.f0), _lib.ptr(lam3), _lib.ptr(torch.zeros_like(lam3)),
                                       *masks, None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
-                                      float(c3.omega), 0.0, _lib.stream_of(c3.f0))
+                                      plane, float(c3.omega), 0.0, _lib.stream_of(c3.f0))
     assert rc != 0 and b"D2Q9 and D3Q27" in _lib.lib().xlbh_last_error()
     # a mask with a solid (255) cell: refused by both entry points, with and without a workspace,
     # at an unaligned mask start (the scan's byte loop) and at an aligned one (its 16-byte loop)
     c = Case("sphere3d", P.FP64FP64)
     st = c.st
-    nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
+    nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
     lam_out, lam_in = c.W.clone(), torch.empty_like(c.W)
     ncell = nx * ny * nz
     for offset in (1, 0):
@@ -287,19 +287,19 @@ def test_adjoint_rejects_bad_arguments():
         args = (st.lattice_id, st._collision_id(), st.precision_policy.store_format, st.compute_id)
         rc = _lib.lib().xlbh_adjoint_step(*args, _lib.ptr(c.f0), _lib.ptr(lam_out), _lib.ptr(lam_in), _lib.ptr(view),
                                           *masks[1:], None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
-                                          float(c.omega), st._collision_param(), _lib.stream_of(c.f0))
+                                          plane, float(c.omega), st._collision_param(), _lib.stream_of(c.f0))
         assert rc != 0 and b"solid" in _lib.lib().xlbh_last_error()
         ws = st.adjoint_workspace(c.bc_mask)
         for workspace, flags in ((None, 0), (ws, _lib.ADJ_BEGIN | _lib.ADJ_END)):
             rc = _lib.lib().xlbh_adjoint_run(*args, _lib.ptr(c.f0), None, _lib.ptr(lam_out), _lib.ptr(lam_in),
                                              _lib.ptr(view), *masks[1:], None, 0, force_mode, fu_ptr, _lib.ptr(ff),
-                                             None, nx, ny, nz, float(c.omega), st._collision_param(), 0, 1,
+                                             None, nx, ny, nz, plane, float(c.omega), st._collision_param(), 0, 1,
                                              _lib.ptr(workspace), flags, _lib.stream_of(c.f0))
             assert rc != 0 and b"solid" in _lib.lib().xlbh_last_error()
         view[ncell - 3] = 0
         rc = _lib.lib().xlbh_adjoint_step(*args, _lib.ptr(c.f0), _lib.ptr(lam_out), _lib.ptr(lam_in), _lib.ptr(view),
                                           *masks[1:], None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
-                                          float(c.omega), st._collision_param(), _lib.stream_of(c.f0))
+                                          plane, float(c.omega), st._collision_param(), _lib.stream_of(c.f0))
         assert rc == 0, _lib.lib().xlbh_last_error()
 
 
@@ -317,11 +317,11 @@ def test_boundary_pass_list_equals_grid_scan(name):
     via_list = torch.empty_like(lam_out)
     st.adjoint_step(c.f0, lam_out, via_list, c.bc_mask, c.missing, c.omega)
     via_scan = torch.empty_like(lam_out)
-    nx, ny, nz, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
+    nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, ff = st._launch_args(c.f0, None, c.bc_mask, c.missing)
     _lib.check(_lib.lib().xlbh_adjoint_step(
         st.lattice_id, st._collision_id(), st.precision_policy.store_format, st.compute_id, _lib.ptr(c.f0),
         _lib.ptr(lam_out), _lib.ptr(via_scan), *masks, None, 0, force_mode, fu_ptr, _lib.ptr(ff), None, nx, ny, nz,
-        float(c.omega), st._collision_param(), _lib.stream_of(c.f0)))
+        plane, float(c.omega), st._collision_param(), _lib.stream_of(c.f0)))
     torch.cuda.synchronize()
     assert st._bc_cells is not None and st._bc_cells.numel() > 0
     err = (via_list - via_scan).abs().max().item()

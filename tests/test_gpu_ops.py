@@ -348,7 +348,7 @@ def test_mass_momentum_reduction(policy):
     out = torch.zeros(4, dtype=torch.float64, device="cuda")
     ws = torch.empty(int(_lib.lib().xlbh_reduce_workspace_bytes(ft[0].numel())), dtype=torch.uint8, device="cuda")
     _lib.check(_lib.lib().xlbh_mass_momentum(1, policy.store_format, _lib.ptr(ft), None, _lib.ptr(out), _lib.ptr(ws),
-                                             *shp, _lib.stream_of(ft)))
+                                             *shp, ft.stride(0), _lib.stream_of(ft)))
     got = out.cpu().numpy()
     np.testing.assert_allclose(got[0], fv.sum(), rtol=1e-13)
     np.testing.assert_allclose(got[1:], np.tensordot(vs.c, fv, axes=(1, 0)).sum(axis=(1, 2, 3)), rtol=1e-10, atol=1e-10)

@@ -1,16 +1,26 @@
 """GPU parity of the fused stepper (xlbh_step / xlbh_run) against the CPU oracle.
 
 Tolerances, per precision:
-* FP32 store and compute: max |drho|, |du| <= 1e-5 after N steps, with no exception, and max |df|
-  <= 2e-6 or the fixed per-case bound in F32_DF_BOUND.  Those bounds hold for cases whose FP32
-  rounding (FMA contraction on the GPU, none in the oracle) is amplified, most by KBC's entropic
-  gamma near equilibrium; each is about twice the largest |df| measured on the GPU.  Where |df|
-  exceeds 2e-6, the oracle's own FP32 error against an FP64 oracle run of the same case is printed
-  beside it as a diagnostic (the noise floor an FP32 CPU code reaches).
-* FP64: max |df| <= 1e-12.
+* FP32 store and compute: max |drho|, |du| <= 1e-5 and max |df| <= 2e-6 after N steps, except for
+  the cases in KBC_AMPLIFIED.  There KBC's entropic gamma (kbc.py:268-296) amplifies rounding
+  differences: near equilibrium its ratio <dS, dH> / <dH, dH> divides rounding noise by rounding
+  noise, and C2's developing wake amplifies any perturbation.  Two correct FP32 codes that round
+  differently (FMA contraction on the GPU, none in the oracle) then disagree by more than 1e-5.  For
+  those cases the test measures the oracle's own FP32 error against an FP64 oracle run, asserts
+  that it exceeds 1e-5 (the amplification is real, not a GPU slip), and asserts that the GPU's
+  FP32 error against the same FP64 run is at most twice the oracle's.  Their FP64 parity is the
+  pin of the algorithm.
+* FP64: max |df| <= 1e-12 (C2 over 500 steps: 1e-10, the same amplification acting on FP64
+  rounding).
 * 16-bit storage: both sides round to 16 bits independently, so stored values may differ by a
   few units in the last place; max distance <= 4 ulps of the stored value, and rho / u bounds
   of a few 16-bit ulps.
+* Block floating point (FP32I16_BLOCK): both sides round to the block format independently, so a
+  value whose unrounded forms straddle a rounding boundary lands one mantissa step apart, and such
+  flips travel on with later steps.  Decoded values within 4 of the field's coarsest mantissa step
+  (largest weight at the smallest segment exponent; measured <= 2); rho / u within that times
+  sqrt(Q); where a segment's largest |f / w - 1| exceeds 2^-10, exponents equal except in at most
+  1 % of those segments, and there by one.
 """
 
 import numpy as np
@@ -34,11 +44,20 @@ TOL = {
     P.FP32BF16: (None, 2e-2),
     P.FP32FP16_SHIFTED: (None, 2e-5),
     P.FP32BF16_SHIFTED: (None, 2e-4),
+    P.FP32I16_BLOCK: (None, 1e-6),
 }
 
-# FP32 cases whose max |df| exceeds 2e-6 (amplified rounding): fixed bounds, about twice the
-# largest value measured on the GPU (case: bound).
-F32_DF_BOUND = {}
+# FP32 cases in which KBC amplifies rounding differences past 1e-5 (module docstring)
+KBC_AMPLIFIED = {"channel2d_regularized", "c2_sphere"}
+_ORACLE_REF = {}
+
+
+def oracle_reference(pair, steps, policy):
+    key = (pair.name, steps, policy)
+    if key not in _ORACLE_REF:
+        _ORACLE_REF[key] = pair.oracle_reference_f(steps, policy)
+    return _ORACLE_REF[key]
+
 
 SIXTEEN = (orc.F16, orc.BF16, orc.F16_SHIFTED, orc.BF16_SHIFTED)
 
@@ -56,7 +75,7 @@ def stored_ulp(raw, fmt):
     return 2.0 ** (np.floor(np.log2(m)) - mant) if m > 0 else 0.0
 
 
-def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None, noise_floor=True):
+def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None):
     pair.step_product(steps, use_run=use_run)
     pair.step_oracle(steps)
     fa, fb = pair.product_f(), pair.oracle_f()
@@ -69,6 +88,38 @@ def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None, noise_
     dr = np.abs(ra - rb)[~solid].max()
     du = np.abs(ua - ub)[:, ~solid].max()
     print(f"parity {pair.name} {pair.policy.name} {steps} steps: max|df| {df:.2e} max|drho| {dr:.2e} max|du| {du:.2e}")
+    if pair.policy == P.FP32FP32 and tol is None and pair.name in KBC_AMPLIFIED:
+        ref = oracle_reference(pair, steps, P.FP64FP64)
+        rr, ur = moments(ref, pair.vs.c)
+        e_o = (np.abs(fb - ref).max(), np.abs(rb - rr)[~solid].max(), np.abs(ub - ur)[:, ~solid].max())
+        e_g = (np.abs(fa - ref).max(), np.abs(ra - rr)[~solid].max(), np.abs(ua - ur)[:, ~solid].max())
+        print(f"  KBC amplification {pair.name}: against the FP64 oracle, oracle FP32 (f, rho, u) "
+              f"{e_o[0]:.2e} {e_o[1]:.2e} {e_o[2]:.2e}; GPU FP32 {e_g[0]:.2e} {e_g[1]:.2e} {e_g[2]:.2e}")
+        assert max(e_o[1], e_o[2]) > 1e-5, f"{pair.name} is listed as amplified but its oracle FP32 error is small"
+        for g, o, what in zip(e_g, e_o, ("f", "rho", "u")):
+            assert g <= 2 * o, f"{pair.name}: GPU FP32 {what} error {g:.3e} > 2 x the oracle's {o:.3e}"
+        return df, dr, du
+    if pair.fmt == orc.I16_BLOCK:
+        from lbm_cases import block_exps
+
+        ka, kb = block_exps(to_np(pair.f0)), block_exps(pair.o_f0)
+        diff = np.flatnonzero(ka != kb)
+        print(f"  block exponents differing in {diff.size} of {ka.size} segments")
+        # Where a segment's largest |f / w - 1| is well above FP32 noise (k <= 24: above 2^-10), the
+        # two sides' maxima differ by a rounding only, so the exponents differ at most by one, where
+        # the maxima straddle a power of two.  In segments at rest the maximum is itself rounding
+        # noise and its exponent is free (those values are w to within 2^-24).
+        active = np.minimum(ka, kb) <= 24
+        assert np.all(np.abs(ka - kb)[active] <= 1) and (np.abs(ka - kb)[active] == 1).sum() <= max(1, active.sum() // 100)
+        # the field's coarsest mantissa step: the largest weight at the smallest exponent
+        qf = float(np.float32(np.max(pair.vs.w))) * 2.0 ** -float(min(ka.min(), kb.min()))
+        bound = ulps
+        print(f"  block: max|df| = {df / qf:.2f} coarse mantissa steps (bound {bound:.1f}), "
+              f"mean {np.abs(fa - fb).mean() / qf:.3f}")
+        assert df <= bound * qf, f"{pair.name} block: max|df| = {df / qf:.2f} mantissa steps > {bound}"
+        tm = max(tm, bound * qf * np.sqrt(pair.vs.q))
+        assert dr <= tm and du <= tm, f"{pair.name} block: max|drho| = {dr:.3e}, max|du| = {du:.3e} > {tm:.1e}"
+        return df, dr, du
     if pair.fmt in SIXTEEN:
         ulp = stored_ulp(pair.o_f0, pair.fmt)
         assert df <= ulps * ulp, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > {ulps} ulps ({ulps * ulp:.3e})"
@@ -78,12 +129,6 @@ def compare(pair, steps, tol=None, use_run=False, ulps=4, mean_ulps=None, noise_
         # rho and u sum Q stored values, each of which may be a few ulps off
         tm = max(tm, 4 * ulp * np.sqrt(pair.vs.q))
     else:
-        if pair.policy == P.FP32FP32 and tol is None:
-            tf = F32_DF_BOUND.get(pair.name, tf)
-        if pair.policy == P.FP32FP32 and df > 2e-6 and noise_floor:
-            ref = pair.oracle_reference_f(steps, P.FP64FP64)
-            print(f"  noise floor {pair.name}: the oracle's own FP32 max|df| against its FP64 run "
-                  f"{np.abs(ref - fb).max():.2e}")
         assert df <= tf, f"{pair.name} {pair.policy}: max|df| = {df:.3e} > {tf:.1e}"
     assert dr <= tm and du <= tm, f"{pair.name} {pair.policy}: max|drho| = {dr:.3e}, max|du| = {du:.3e} > {tm:.1e}"
     return df, dr, du
@@ -111,16 +156,19 @@ def test_c1_cavity64_1000_steps_matches_oracle(policy):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+@pytest.mark.parametrize("policy", [P.FP64FP64, P.FP32FP32])
 def test_c2_500_steps_matches_oracle(policy):
     """C2 at its full size (D3Q27 KBC wind tunnel 256x128x128, sphere r = 12, Re = 200): 500 steps
-    (the flow around the sphere is developing) against the oracle in FP32 and FP64, with the
-    stepper tolerances above (rho, u <= 1e-5 in FP32; f <= 1e-12 in FP64)."""
-    compare(Pair("c2_sphere", policy), 500, use_run=True, noise_floor=False)
+    (the flow around the sphere is developing) against the oracle.  FP64: f, rho, u <= 1e-10 (the
+    wake amplifies FP64 rounding differences from 1e-16 to ~5e-11 over 500 steps).  FP32: the
+    KBC-amplified comparison of the module docstring (GPU error against the FP64 oracle at most
+    twice the oracle's own FP32 error)."""
+    tol = (1e-10, 1e-10) if policy == P.FP64FP64 else None
+    compare(Pair("c2_sphere", policy), 500, tol=tol, use_run=True)
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED])
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32I16_BLOCK])
 def test_c3_full_size_matches_oracle(policy):
     """C3 at its full size (periodic Taylor-Green vortex 512x256x256): 10 steps against the
     oracle in FP32, FP64 and shifted FP16 / BF16 storage.  For 16-bit storage the two sides round
@@ -129,16 +177,16 @@ def test_c3_full_size_matches_oracle(policy):
     so the elementwise bound is 8 ulps of the field's largest stored magnitude, with the mean
     difference below a quarter of that ulp (measured 0.14: most values are far smaller than the
     largest)."""
-    compare(Pair("c3_tgv", policy), 10, ulps=8, mean_ulps=0.25, noise_floor=False)
+    compare(Pair("c3_tgv", policy), 10, ulps=8, mean_ulps=0.25)
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64])
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP64FP64, P.FP32I16_BLOCK])
 def test_mlups_cavity256_matches_oracle(policy):
     """The benchmark's own workload (mlups_3d.py:45-90 at edge 256: D3Q19 BGK, EquilibriumBC lid
     u = (0.02, 0, 0) on the top face without edges, FullwayBounceBackBC on the other five faces,
     omega = 1, rest start): 30 steps through the C loop against the oracle."""
-    compare(Pair("mlups_cavity256", policy), 30, use_run=True, noise_floor=False)
+    compare(Pair("mlups_cavity256", policy), 30, use_run=True)
 
 
 @pytest.mark.parametrize("name", ["periodic", "cavity_fullway", "channel2d_zouhe", "channel2d_regularized", "sphere3d",
@@ -153,9 +201,32 @@ def test_stepper_reduced_storage_matches_oracle(name, policy):
     compare(Pair(name, policy), 20)
 
 
-def test_initial_state_matches_oracle():
-    pair = Pair("cavity_fullway", P.FP32FP32)
+# Cases of the block format against the block oracle: every lattice and collision, both segment
+# regimes (one segment per row and several), every boundary kind (prescribed values of Zou-He and
+# regularized BCs from the side table on both sides: an int16 mantissa cannot carry the
+# reference's f_1[0] stash).
+BLOCK_CASES = ["periodic", "periodic_kbc", "cavity_halfway", "cavity_fullway", "sphere3d", "sphere3d_grads",
+               "channel2d_zouhe", "channel2d_zouhe_pressure",
+               "cavity_smagorinsky", "do_nothing", "periodic_field", "tiny_periodic2d", "tiny_cavity2d",
+               "tiny_cavity3d", "mlups_cavity64", "long_periodic3d", "long_periodic2d_kbc", "long_cavity3d",
+               "long_cavity2d"]
+
+
+@pytest.mark.parametrize("name", BLOCK_CASES)
+def test_stepper_block_matches_oracle(name):
+    compare(Pair(name, P.FP32I16_BLOCK), 40)
+
+
+@pytest.mark.parametrize("policy", [P.FP32FP32, P.FP32I16_BLOCK])
+def test_initial_state_matches_oracle(policy):
+    """The rest state (f = w exactly) and the masks, bit for bit; for the block format also every
+    segment exponent."""
+    pair = Pair("cavity_fullway", policy)
     np.testing.assert_array_equal(to_np(pair.f0), pair.o_f0)
+    if policy == P.FP32I16_BLOCK:
+        from lbm_cases import block_exps
+
+        np.testing.assert_array_equal(block_exps(to_np(pair.f0)), block_exps(pair.o_f0))
     np.testing.assert_array_equal(pair.bc_mask.cpu().numpy(), pair.o_mask)
     np.testing.assert_array_equal(pair.missing.cpu().numpy().astype(np.uint8), pair.o_missing)
 
@@ -213,7 +284,7 @@ def test_periodic_full_size_conserves_mass():
     def mm(f):
         out = torch.zeros(4, dtype=torch.float64, device="cuda")
         _lib.check(_lib.lib().xlbh_mass_momentum(1, _lib.F64, _lib.ptr(f), None, _lib.ptr(out), _lib.ptr(ws), 128, 128, 128,
-                                                 _lib.stream_of(f)))
+                                                 f.stride(0), _lib.stream_of(f)))
         return out.cpu().numpy()
 
     m0 = mm(f0)

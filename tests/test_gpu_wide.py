@@ -75,7 +75,7 @@ def _run(stepper, f_a, f_b, masks, nx, omega, steps):
     _lib.check(
         _lib.lib().xlbh_run(
             stepper.lattice_id, _lib.BGK, stepper.precision_policy.store_format, stepper.compute_id,
-            _lib.ptr(f_a), _lib.ptr(f_b), *masks, _lib.FORCE_NONE, fu, None, nx, NY, NZ, float(omega), 0.0, 0,
+            _lib.ptr(f_a), _lib.ptr(f_b), *masks, _lib.FORCE_NONE, fu, None, nx, NY, NZ, f_a.stride(0), float(omega), 0.0, 0,
             int(steps), _lib.stream_of(f_a),
         ),
         "xlbh_run",

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libxlbhip.so")
 # ABI constants (include/xlb_hip.h)
 D2Q9, D3Q19, D3Q27 = 0, 1, 2
 BGK, KBC, SMAGORINSKY_BGK = 0, 1, 2
-F32, F64, F16, BF16, F16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3, 4, 5
+F32, F64, F16, BF16, F16_SHIFTED, BF16_SHIFTED, I16_BLOCK = 0, 1, 2, 3, 4, 5, 6
 COMPUTE_F32, COMPUTE_F64 = 0, 1
 FIELD_F32, FIELD_F64, FIELD_F16, FIELD_BF16 = 0, 1, 2, 3
 BC_NONE = 0
@@ -33,7 +33,7 @@ BC_GRADS_APPROXIMATION = 10
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 ADJ_RECOMPUTE, ADJ_BEGIN, ADJ_END = 1, 2, 4
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class BcDesc(ctypes.Structure):
@@ -58,13 +58,15 @@ _bcp = ctypes.POINTER(BcDesc)
 SIGNATURES = {
     "xlbh_last_error": (ctypes.c_char_p, []),
     "xlbh_abi_version": (_i, []),
-    "xlbh_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _d, _d, _i, _vp]),
-    "xlbh_run": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _d, _d, _i, _i, _vp]),
-    "xlbh_stream": (_i, [_i, _i, _vp, _vp, _i, _i, _i, _vp]),
-    "xlbh_equilibrium": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp]),
-    "xlbh_moments": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
-    "xlbh_collide": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _dp, _vp, _i, _i, _i, _vp]),
-    "xlbh_apply_bc": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "xlbh_plane_stride": (_i64, [_i, _i, _i, _i, _i]),
+    "xlbh_population_bytes": (_i64, [_i, _i, _i, _i, _i, _i64]),
+    "xlbh_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _i64, _d, _d, _i, _vp]),
+    "xlbh_run": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _i64, _d, _d, _i, _i, _vp]),
+    "xlbh_stream": (_i, [_i, _i, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_equilibrium": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_moments": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_collide": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _d, _i, _dp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_apply_bc": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_mask_indices": (_i, [_i, _vp, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_pack_missing": (_i, [_i, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_compact_workspace_bytes": (_i64, [_i64]),
@@ -76,14 +78,14 @@ SIGNATURES = {
     "xlbh_q_criterion": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_mesh_mask": (_i, [_i, _vp, _i64, _i, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_reduce_workspace_bytes": (_i64, [_i64]),
-    "xlbh_momentum_transfer": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
-    "xlbh_mass_momentum": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "xlbh_momentum_transfer": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_mass_momentum": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_adjoint_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
-                               _i, _d, _d, _vp]),
+                               _i, _i64, _d, _d, _vp]),
     "xlbh_adjoint_run": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i,
-                              _i, _i, _d, _d, _i, _i, _vp, _i, _vp]),
+                              _i, _i, _i64, _d, _d, _i, _i, _vp, _i, _vp]),
     "xlbh_adjoint_workspace_bytes": (_i64, [_i, _i, _i, _i, _i]),
-    "xlbh_record": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _d, _d, _i, _i, _vp]),
+    "xlbh_record": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _i64, _d, _d, _i, _i, _vp]),
 }
 
 _LIB = None

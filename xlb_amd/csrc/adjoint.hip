@@ -91,7 +91,7 @@ static AdjParams adjoint_params(int collision, double cparam, const void* f_in, 
                                 const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
                                 const xlbh_bc_desc* bc_table, const int32_t* bc_cells, int n_bc_cells, int force_mode,
                                 const double* force_uniform, const void* force_field, double* grad_force, int nx,
-                                int ny, int nz, double omega) {
+                                int ny, int nz, uint64_t plane, double omega) {
   AdjParams p;
   p.collision = collision;
   p.cparam = cparam;
@@ -115,6 +115,7 @@ static AdjParams adjoint_params(int collision, double cparam, const void* f_in, 
   p.omega = omega;
   p.sw = make_sweep(nx, ny, nz);
   p.N = (uint32_t)((int64_t)nx * ny * nz);
+  p.P = plane;
   return p;
 }
 
@@ -122,13 +123,16 @@ extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compu
                                 void* lam_a, void* lam_b, const uint8_t* bc_mask, const uint32_t* missing_bits,
                                 const void* aux, const xlbh_bc_desc* bc_table, const int32_t* bc_cells,
                                 int n_bc_cells, int force_mode, const double* force_uniform,
-                                const void* force_field, double* grad_force, int nx, int ny, int nz, double omega,
-                                double collision_param, int timestep0, int n_steps, void* workspace, int flags,
-                                void* stream) {
+                                const void* force_field, double* grad_force, int nx, int ny, int nz, int64_t plane,
+                                double omega, double collision_param, int timestep0, int n_steps, void* workspace,
+                                int flags, void* stream) {
   clear_error();
   if (int e = adjoint_validate(lattice, collision, force_mode, force_uniform, force_field, bc_mask, missing_bits,
                                bc_table, nx, ny, nz))
     return e;
+  XLBH_REQUIRE(store != XLBH_I16_BLOCK, "the adjoint does not read the block format");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   XLBH_REQUIRE(n_steps >= 1, "n_steps must be >= 1");
   XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
   XLBH_REQUIRE(f_ckpt && lam_a && lam_b, "adjoint_run needs f_ckpt, lam_a and lam_b");
@@ -139,7 +143,7 @@ extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compu
   // every call without one
   if (!workspace || (flags & XLBH_ADJ_BEGIN))
     if (int e = refuse_solid_cells(bc_mask, nx, ny, nz, (hipStream_t)stream)) return e;
-  const size_t field_bytes = (size_t)lattice_q(lattice) * (size_t)nx * ny * nz * store_bytes(store);
+  const size_t field_bytes = (size_t)lattice_q(lattice) * P * store_bytes(store);
   auto state = [&](int j) -> const void* {
     return j == 0 ? f_ckpt : static_cast<const void*>(static_cast<const char*>(states) + (size_t)(j - 1) * field_bytes);
   };
@@ -147,8 +151,8 @@ extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compu
   const bool recompute = (flags & XLBH_ADJ_RECOMPUTE) != 0;
   for (int j = 1; recompute && j < n_steps; ++j) {
     if (int e = xlbh_step(lattice, collision, store, compute, state(j - 1), const_cast<void*>(state(j)), bc_mask,
-                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz, omega,
-                          collision_param, timestep0 + j - 1, stream))
+                          missing_bits, aux, bc_table, force_mode, force_uniform, force_field, nx, ny, nz,
+                          (int64_t)P, omega, collision_param, timestep0 + j - 1, stream))
       return e;
   }
   void* cur = lam_a;
@@ -161,7 +165,7 @@ extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compu
   uint8_t* flag = ws ? reinterpret_cast<uint8_t*>(ws + 3 * qn * cb) : nullptr;
   auto fused_params = [&](const void* f, const void* lo, void* li, int t) {
     AdjParams p = adjoint_params(collision, collision_param, f, lo, li, bc_mask, missing_bits, aux, bc_table, bc_cells,
-                                 n_bc_cells, force_mode, force_uniform, force_field, grad_force, nx, ny, nz, omega);
+                                 n_bc_cells, force_mode, force_uniform, force_field, grad_force, nx, ny, nz, P, omega);
     p.d_own = ws;
     p.d_nb_out = ws + (1 + (t & 1)) * qn * cb;
     p.d_nb_in = ws + (1 + ((t + 1) & 1)) * qn * cb;
@@ -180,7 +184,7 @@ extern "C" int xlbh_adjoint_run(int lattice, int collision, int store, int compu
     } else {
       const AdjParams p = adjoint_params(collision, collision_param, state(j), cur, nxt, bc_mask, missing_bits, aux,
                                          bc_table, bc_cells, n_bc_cells, force_mode, force_uniform, force_field,
-                                         grad_force, nx, ny, nz, omega);
+                                         grad_force, nx, ny, nz, P, omega);
       if (int e = adjoint_dispatch(lattice, store, compute, p, s)) return e;
     }
     void* t = cur;
@@ -206,17 +210,20 @@ extern "C" int xlbh_adjoint_step(int lattice, int collision, int store, int comp
                                  const uint32_t* missing_bits, const void* aux, const xlbh_bc_desc* bc_table,
                                  const int32_t* bc_cells, int n_bc_cells, int force_mode,
                                  const double* force_uniform, const void* force_field, double* grad_force, int nx,
-                                 int ny, int nz, double omega, double collision_param, void* stream) {
+                                 int ny, int nz, int64_t plane, double omega, double collision_param, void* stream) {
   clear_error();
   if (int e = adjoint_validate(lattice, collision, force_mode, force_uniform, force_field, bc_mask, missing_bits,
                                bc_table, nx, ny, nz))
     return e;
+  XLBH_REQUIRE(store != XLBH_I16_BLOCK, "the adjoint does not read the block format");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   XLBH_REQUIRE(f_in && lam_out && lam_in, "adjoint_step needs f_in, lam_out and lam_in");
   XLBH_REQUIRE(lam_in != lam_out, "lam_in must not alias lam_out");
   XLBH_REQUIRE(n_bc_cells >= 0 && (n_bc_cells == 0 || bc_cells), "bc_cells is null with n_bc_cells > 0");
   if (int e = refuse_solid_cells(bc_mask, nx, ny, nz, (hipStream_t)stream)) return e;
   const AdjParams p = adjoint_params(collision, collision_param, f_in, lam_out, lam_in, bc_mask, missing_bits, aux,
                                      bc_table, bc_cells, n_bc_cells, force_mode, force_uniform, force_field,
-                                     grad_force, nx, ny, nz, omega);
+                                     grad_force, nx, ny, nz, P, omega);
   return adjoint_dispatch(lattice, store, compute, p, (hipStream_t)stream);
 }

@@ -418,6 +418,7 @@ struct AdjParams {
   int collision;   // XLBH_BGK or XLBH_SMAGORINSKY_BGK
   Sweep sw;
   uint32_t N;
+  uint64_t P;  // plane stride (elements) of f_in; the adjoint fields are dense (Q, N)
 };
 
 // The populations the forward step pulled into cell (x, y, z), f_ps (periodic wrap).
@@ -431,7 +432,7 @@ __device__ __forceinline__ void adj_pull(const AdjParams& p, int x, int y, int z
 #pragma unroll
   for (int l = 0; l < Q; ++l) {
     const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
-    fps[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
+    fps[l] = ld_pop<FMT, C>(fin + (size_t)l * p.P + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
   }
 }
 
@@ -460,16 +461,16 @@ __device__ __forceinline__ void adj_prepare_pulled(const AdjParams& p, uint32_t 
   } else if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
 #pragma unroll
     for (int l = 0; l < Q; ++l)
-      if (bit(mm, l)) h[l] = ld_pop<FMT, C>(fin + (size_t)T.opp[l] * p.N + cell, T.w[T.opp[l]]);
+      if (bit(mm, l)) h[l] = ld_pop<FMT, C>(fin + (size_t)T.opp[l] * p.P + cell, T.w[T.opp[l]]);
   } else if (kind == XLBH_BC_DO_NOTHING) {
 #pragma unroll
-    for (int l = 0; l < Q; ++l) h[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + cell, T.w[l]);
+    for (int l = 0; l < Q; ++l) h[l] = ld_pop<FMT, C>(fin + (size_t)l * p.P + cell, T.w[l]);
   } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
     const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
     zouhe_cell<LAT>(h, mm, pv, kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY,
                     kind >= XLBH_BC_REGULARIZED_VELOCITY);
   } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
-    grads_cell<LAT>(h, mm, [&](int l) { return ld_pop<FMT, C>(fin + (size_t)l * p.N + cell, T.w[l]); });
+    grads_cell<LAT>(h, mm, [&](int l) { return ld_pop<FMT, C>(fin + (size_t)l * p.P + cell, T.w[l]); });
   }
   if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
 #pragma unroll
@@ -516,7 +517,7 @@ __device__ __forceinline__ void own_pre(const AdjParams& p, uint32_t cell, C* pr
   constexpr LatTables T = make_tables(LAT);
   const typename Fmt<FMT>::raw* fin = static_cast<const typename Fmt<FMT>::raw*>(p.f_in);
 #pragma unroll
-  for (int l = 0; l < T.q; ++l) pre[l] = ld_pop<FMT, C>(fin + (size_t)l * p.N + cell, T.w[l]);
+  for (int l = 0; l < T.q; ++l) pre[l] = ld_pop<FMT, C>(fin + (size_t)l * p.P + cell, T.w[l]);
 }
 
 // Kernel A: every cell pushes G_ps into lam_in (a permutation) and adds its force gradient.

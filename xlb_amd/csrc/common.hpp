@@ -161,6 +161,13 @@ struct Fmt<XLBH_BF16_SHIFTED> {
   using raw = uint16_t;
   static constexpr bool shifted = true;
 };
+// 16-bit block floating point: int16 mantissas m of x = f_l / w_l - 1 with one exponent k per row
+// segment (blk_* below), f_l = w_l + m * w_l * 2^-k.
+template <>
+struct Fmt<XLBH_I16_BLOCK> {
+  using raw = int16_t;
+  static constexpr bool shifted = false;
+};
 
 template <class R, class C>
 __device__ __forceinline__ C raw_to(R v) {
@@ -292,12 +299,129 @@ __device__ __forceinline__ int wrap(int v, int n) {
   v %= n;
   return v < 0 ? v + n : v;
 }
+// wrap() for v in [-1, n]
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 inline bool valid_lattice(int lat) { return lat == XLBH_D2Q9 || lat == XLBH_D3Q19 || lat == XLBH_D3Q27; }
-inline bool valid_store(int s) { return s >= XLBH_F32 && s <= XLBH_BF16_SHIFTED; }
+inline bool valid_store(int s) { return s >= XLBH_F32 && s <= XLBH_I16_BLOCK; }
 inline int lattice_q(int lat) { return lat == XLBH_D2Q9 ? 9 : (lat == XLBH_D3Q19 ? 19 : 27); }
 inline size_t store_bytes(int s) { return s == XLBH_F64 ? 8 : (s == XLBH_F32 ? 4 : 2); }
 
+// ------------------------------------------------------------------------------------------
+// Block floating point (XLBH_I16_BLOCK).  A row segment is the run of bz cells along the fast
+// axis that one sweep tile covers: 3-D (x, y, z >> lbz), 2-D (x, y >> lbz).  Each segment holds
+// one int32 exponent k, shared by every direction; the exponents of a field follow its last
+// plane, at byte offset blk_exps_offset(q, plane).  k is chosen from the largest
+// |x| = |f_l / w_l - 1| over the segment's written cells and directions, M: with
+// M' = M * (1 + 2^-15) rounded to float and E its unbiased exponent, k = 14 - E clamped to
+// [kBlkKMin, kBlkKMax], so every |m| <= 32767.  m = rint_even(t) with t = fma(f, 2^k / w_l, -2^k)
+// rounded to float (1 / w_l is exact in float for every lattice weight).
+// ------------------------------------------------------------------------------------------
+constexpr int kBlkKMin = -64;
+constexpr int kBlkKMax = 60;
+// Exponent of a segment whose largest |x| is not finite (int16 mantissas cannot hold NaN or
+// infinity): its cells are stored with m = 32767, which this exponent decodes to +inf, so a
+// diverging run stays visible.
+constexpr int kBlkNonFinite = -127;
+
+__host__ __device__ inline uint32_t blk_segs_fast(const Sweep& s) {
+  const uint32_t fast = (s.nz > 1) ? (uint32_t)s.nz : (uint32_t)s.ny;
+  return (fast + (1u << s.lbz) - 1) >> s.lbz;
+}
+__host__ __device__ inline uint64_t blk_seg_count(const Sweep& s) {
+  const uint64_t rows = (s.nz > 1) ? (uint64_t)s.nx * s.ny : (uint64_t)s.nx;
+  return rows * blk_segs_fast(s);
+}
+// Segment of cell (x, y, z).
+__host__ __device__ inline uint32_t blk_seg(const Sweep& s, int x, int y, int z) {
+  if (s.nz > 1) return ((uint32_t)x * s.ny + y) * blk_segs_fast(s) + ((uint32_t)z >> s.lbz);
+  return (uint32_t)x * blk_segs_fast(s) + ((uint32_t)y >> s.lbz);
+}
+// Segment fb of row (xr, yr) (3-D) or row xr (2-D).
+__host__ __device__ inline uint32_t blk_row_seg(const Sweep& s, int xr, int yr, int fb) {
+  if (s.nz > 1) return ((uint32_t)xr * s.ny + yr) * blk_segs_fast(s) + fb;
+  return (uint32_t)xr * blk_segs_fast(s) + fb;
+}
+__host__ __device__ inline size_t blk_exps_offset(int q, size_t plane) { return ((size_t)q * plane * 2 + 255) / 256 * 256; }
+
+__host__ __device__ inline uint32_t f2u(float f) {
+  uint32_t b;
+  __builtin_memcpy(&b, &f, 4);
+  return b;
+}
+__host__ __device__ inline float u2f(uint32_t b) {
+  float f;
+  __builtin_memcpy(&f, &b, 4);
+  return f;
+}
+
+// Exponent from the segment's largest |x|, m >= 0 or NaN.
+__host__ __device__ inline int blk_k_from_max(float m) {
+  if (!(m <= 3.0e38f)) return kBlkNonFinite;  // NaN or infinite
+  const float mp = m * 1.000030517578125f;    // (1 + 2^-15), one rounding
+  const int e = (int)((f2u(mp) >> 23) & 0xffu) - 127;
+  const int k = 14 - e;
+  return k < kBlkKMin ? kBlkKMin : (k > kBlkKMax ? kBlkKMax : k);
+}
+// 2^k (exact, normal for k in [kBlkKMin, kBlkKMax]).
+__host__ __device__ inline float blk_pow2(int k) { return u2f((uint32_t)(k + 127) << 23); }
+// Decode scale w * 2^-k (exact): f = fma(m, scale, w).  For kBlkNonFinite the scale exceeds
+// 2^119, so m = 32767 decodes to +inf.
+__host__ __device__ inline float blk_dscale(float w, int k) { return u2f(f2u(w) - ((uint32_t)k << 23)); }
+// Encode scale 2^k / w (exact; 1/w is a float for every lattice weight): t = fma(f, scale, -2^k),
+// m = rint_even(t).
+__host__ __device__ inline float blk_escale(float inv_w, int k) { return u2f(f2u(inv_w) + ((uint32_t)k << 23)); }
+// rint_even(t) as int16 for |t| < 2^15: t + 1.5 * 2^23 lies in [2^23, 2^24), where the float
+// spacing is 1, so the addition rounds t to an integer held in the low mantissa bits.
+__host__ __device__ inline int16_t blk_round(float t) { return (int16_t)(f2u(t + 12582912.0f) & 0xffffu); }
+
+// Largest of v over the wave's 64 lanes (all lanes active), as a uniform value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xf, false);
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, dpp_u32<0xb1, 0xf>(v));   // quad_perm [1,0,3,2]
+  v = max(v, dpp_u32<0x4e, 0xf>(v));   // quad_perm [2,3,0,1]
+  v = max(v, dpp_u32<0x141, 0xf>(v));  // row_half_mirror
+  v = max(v, dpp_u32<0x140, 0xf>(v));  // row_mirror: each 16-lane row holds its maximum
+  v = max(v, dpp_u32<0x142, 0xa>(v));  // row_bcast:15 into rows 1 and 3
+  v = max(v, dpp_u32<0x143, 0xc>(v));  // row_bcast:31 into rows 2 and 3: lane 63 holds the maximum
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Largest |f_l / w_l - 1| of one cell (NaN if any is NaN).
+template <int LAT>
+__device__ __forceinline__ float blk_cell_max(const float* f) {
+  constexpr LatTables T = make_tables(LAT);
+  float a = 0.0f;
+#pragma unroll
+  for (int l = 0; l < T.q; ++l)
+    a = __builtin_elementwise_maximum(a, __builtin_fabsf(__builtin_fmaf(f[l], (float)(1.0 / T.w[l]), -1.0f)));
+  return a;
+}
+
+// Exponent of the caller's segment from each lane's largest |x| (0 for lanes that write
+// nothing).  Every thread of the block must call it: for segments of 128 or 256 cells it
+// synchronises the block.
+__device__ __forceinline__ int blk_segment_k(float amax, int lbz) {
+  __shared__ uint32_t red[4];
+  // |x| >= 0, so the float order is the order of the bit patterns; NaN sorts above infinity.
+  uint32_t b = wave_max_u32(f2u(amax));
+  if (lbz > 6) {
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wave] = b;
+    __syncthreads();
+    const int first = wave & ~((1 << (lbz - 6)) - 1);
+    b = max(red[first], red[first + 1]);
+    if (lbz > 7) b = max(b, max(red[first + 2], red[first + 3]));
+  }
+  return blk_k_from_max(u2f(b));
+}
+
 int check_grid(int lattice, int nx, int ny, int nz);
+// Plane stride (elements) of a population field: 0 selects the dense stride N; otherwise
+// plane >= N.  Returns 0 and sets `out`, or non-zero with the error set.
+int resolve_plane(int lattice, int store, int nx, int ny, int nz, int64_t plane, uint64_t& out);
 
 }  // namespace xlbh

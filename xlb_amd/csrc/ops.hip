@@ -1,26 +1,33 @@
 // Standalone operators: the reference's per-operator Warp kernels (used on their own by the
 // operator API and its tests; the stepper uses the fused kernel of step.hpp instead).
-#include "physics.hpp"
+#include "step.hpp"
 
 namespace xlbh {
 
 // ------------------------------------------------------------------------------------------
 // Stream: f_out[l](x) = f_in[l](x - c_l), periodic (stream.py:59-99).  A pure permutation of
-// stored values, so raw elements are moved without conversion.
+// stored values, so raw elements are moved without conversion.  It is the step kernel's memory
+// pattern without its arithmetic (same addressing, launch shape and occupancy, the same Q pulls
+// issued back to back and non-temporal stores), so its bandwidth is the ceiling the step's
+// access pattern allows.
 // ------------------------------------------------------------------------------------------
-template <int LAT, class R>
+template <int LAT, class R, bool WIDE>
 __global__ __launch_bounds__(256) void k_stream(const R* __restrict__ fin, R* __restrict__ fout, Sweep sw,
-                                                uint32_t N) {
+                                                uint64_t P) {
   constexpr LatTables T = make_tables(LAT);
+  using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
   int x, y, z;
   if (!sweep_cell(sw, x, y, z)) return;
   const int nx = sw.nx, ny = sw.ny, nz = sw.nz;
-  const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
+  const O cell = ((O)x * ny + y) * nz + z;
+  const size_t pb = (size_t)P * sizeof(R);
+  const Planes<R, O> in{reinterpret_cast<const char*>(fin), pb}, out{reinterpret_cast<const char*>(fout), pb};
+  const PullOffsets<O> po(x, y, z, nx, ny, nz);
+  R v[T.q];
 #pragma unroll
-  for (int l = 0; l < T.q; ++l) {
-    const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
-    fout[(size_t)l * N + cell] = fin[(size_t)l * N + ((uint32_t)sx * ny + sy) * nz + sz];
-  }
+  for (int l = 0; l < T.q; ++l) v[l] = ld_stream(in.at(l, po.template src<LAT>(l)));
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) st_stream(out.at_w(l, cell), v[l]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -29,18 +36,57 @@ __global__ __launch_bounds__(256) void k_stream(const R* __restrict__ fin, R* __
 template <int LAT, int FMT, class C, class FR>
 __global__ __launch_bounds__(256) void k_equilibrium(const FR* __restrict__ rho, const FR* __restrict__ u,
                                                      typename Fmt<FMT>::raw* __restrict__ f, Sweep sw,
-                                                     uint32_t N) {
+                                                     uint32_t N, uint64_t P) {
   constexpr LatTables T = make_tables(LAT);
+  constexpr bool BLK = FMT == XLBH_I16_BLOCK;
   int x, y, z;
-  if (!sweep_cell(sw, x, y, z)) return;
+  const bool live = sweep_cell(sw, x, y, z);
+  if constexpr (!BLK) {
+    if (!live) return;
+  } else {
+    // every lane of the block stays to the segment reduction; lanes outside the grid read a cell
+    // of their own row and write nothing
+    x = min(x, sw.nx - 1);
+    y = min(y, sw.ny - 1);
+    z = min(z, sw.nz - 1);
+  }
   const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
   C uu[3] = {C(0), C(0), C(0)};
 #pragma unroll
   for (int d = 0; d < T.d; ++d) uu[d] = raw_to<FR, C>(u[(size_t)d * N + cell]);
   const C r = raw_to<FR, C>(rho[cell]);
   const C us = usqr15<LAT>(uu);
+  C fe[T.q];
 #pragma unroll
-  for (int l = 0; l < T.q; ++l) st_pop<FMT, C>(f + (size_t)l * N + cell, feq_l<LAT>(l, r, uu, us), T.w[l]);
+  for (int l = 0; l < T.q; ++l) fe[l] = feq_l<LAT>(l, r, uu, us);
+  if constexpr (!BLK) {
+#pragma unroll
+    for (int l = 0; l < T.q; ++l) st_pop<FMT, C>(f + (size_t)l * P + cell, fe[l], T.w[l]);
+  } else {
+    const int k = blk_segment_k(live ? blk_cell_max<LAT>(fe) : 0.0f, sw.lbz);
+    int32_t* e = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(f) + blk_exps_offset(T.q, P));
+    if ((threadIdx.x & ((1u << sw.lbz) - 1)) == 0) e[blk_seg(sw, x, y, z)] = k;
+    if (live) {
+#pragma unroll
+      for (int l = 0; l < T.q; ++l)
+        f[(size_t)l * P + cell] = k == kBlkNonFinite
+                                      ? (int16_t)32767
+                                      : blk_round(__builtin_fmaf(fe[l], blk_escale((float)(1.0 / T.w[l]), k),
+                                                                 -blk_pow2(k)));
+    }
+  }
+}
+
+// Population l of `cell` from field f (plane stride P) as compute type C; for the block format
+// with exponents e, the cell's segment exponent is looked up.
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ C ld_pop_at(const typename Fmt<FMT>::raw* f, uint64_t P, const int32_t* e, int k, int l,
+                                       uint32_t cell) {
+  constexpr LatTables T = make_tables(LAT);
+  if constexpr (FMT == XLBH_I16_BLOCK)
+    return (C)__builtin_fmaf((float)f[(size_t)l * P + cell], blk_dscale((float)T.w[l], k), (float)T.w[l]);
+  else
+    return ld_pop<FMT, C>(f + (size_t)l * P + cell, T.w[l]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -49,14 +95,16 @@ __global__ __launch_bounds__(256) void k_equilibrium(const FR* __restrict__ rho,
 // ------------------------------------------------------------------------------------------
 template <int LAT, int FMT, class C, class FR>
 __global__ __launch_bounds__(256) void k_moments(const typename Fmt<FMT>::raw* __restrict__ f, FR* rho,
-                                                 FR* u, FR* pi, int which, Sweep sw, uint32_t N) {
+                                                 FR* u, FR* pi, int which, Sweep sw, uint32_t N, uint64_t P) {
   constexpr LatTables T = make_tables(LAT);
   int x, y, z;
   if (!sweep_cell(sw, x, y, z)) return;
   const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
+  const int32_t* e = reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(f) + blk_exps_offset(T.q, P));
+  const int k = FMT == XLBH_I16_BLOCK ? e[blk_seg(sw, x, y, z)] : 0;
   C ff[T.q];
 #pragma unroll
-  for (int l = 0; l < T.q; ++l) ff[l] = ld_pop<FMT, C>(f + (size_t)l * N + cell, T.w[l]);
+  for (int l = 0; l < T.q; ++l) ff[l] = ld_pop_at<LAT, FMT, C>(f, P, e, k, l, cell);
   if (which == 3) {
 #pragma unroll
     for (int t = 0; t < T.nt; ++t) {
@@ -105,7 +153,7 @@ __global__ __launch_bounds__(256) void k_collide(const typename Fmt<FMT>::raw* _
                                                  const FR* __restrict__ rho, const FR* __restrict__ u,
                                                  C omega, C cparam, int force_mode, double fu0, double fu1,
                                                  double fu2, const C* __restrict__ force_field, Sweep sw,
-                                                 uint32_t N) {
+                                                 uint32_t N, uint64_t P) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   int x, y, z;
@@ -114,8 +162,8 @@ __global__ __launch_bounds__(256) void k_collide(const typename Fmt<FMT>::raw* _
   C ff[Q], fe[Q];
 #pragma unroll
   for (int l = 0; l < Q; ++l) {
-    ff[l] = ld_pop<FMT, C>(f + (size_t)l * N + cell, T.w[l]);
-    fe[l] = ld_pop<FMT, C>(feq + (size_t)l * N + cell, T.w[l]);
+    ff[l] = ld_pop<FMT, C>(f + (size_t)l * P + cell, T.w[l]);
+    fe[l] = ld_pop<FMT, C>(feq + (size_t)l * P + cell, T.w[l]);
   }
   C out[Q];
   C r = C(0), uu[3] = {C(0), C(0), C(0)};
@@ -191,7 +239,7 @@ __global__ __launch_bounds__(256) void k_collide(const typename Fmt<FMT>::raw* _
     for (int l = 0; l < Q; ++l) out[l] += feq_l<LAT>(l, r, uf, usf) - fe[l];
   }
 #pragma unroll
-  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout + (size_t)l * N + cell, out[l], T.w[l]);
+  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout + (size_t)l * P + cell, out[l], T.w[l]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -205,7 +253,7 @@ __global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc
                                                   typename Fmt<FMT>::raw* fpost,
                                                   const uint8_t* __restrict__ bc_mask,
                                                   const uint32_t* __restrict__ missing, const C* aux,
-                                                  Sweep sw, uint32_t N) {
+                                                  Sweep sw, uint64_t P) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   int x, y, z;
@@ -215,8 +263,8 @@ __global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc
   const uint32_t mm = missing ? missing[cell] : 0u;
   C f[Q];
 #pragma unroll
-  for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fpost + (size_t)l * N + cell, T.w[l]);
-  auto pre = [&](int l) { return ld_pop<FMT, C>(fpre + (size_t)l * N + cell, T.w[l]); };
+  for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fpost + (size_t)l * P + cell, T.w[l]);
+  auto pre = [&](int l) { return ld_pop<FMT, C>(fpre + (size_t)l * P + cell, T.w[l]); };
   if (kind == XLBH_BC_EQUILIBRIUM) {
     const C rb = (C)desc.rho;
     const C ub[3] = {(C)desc.u[0], (C)desc.u[1], (C)desc.u[2]};
@@ -241,7 +289,7 @@ __global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc
     grads_cell<LAT>(f, mm, pre);
   }
 #pragma unroll
-  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fpost + (size_t)l * N + cell, f[l], T.w[l]);
+  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fpost + (size_t)l * P + cell, f[l], T.w[l]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -282,6 +330,17 @@ int with_precision(int store, int compute, Fn&& fn) {
   return 1;
 }
 
+// with_precision plus the block format (FP32 compute), for the operators that support it.
+template <class Fn>
+int with_precision_blk(int store, int compute, Fn&& fn) {
+  if (store == XLBH_I16_BLOCK) {
+    if (compute == XLBH_COMPUTE_F32) return fn(std::integral_constant<int, XLBH_I16_BLOCK>{}, float{});
+    set_error("the block format (XLBH_I16_BLOCK) computes in FP32 only");
+    return 1;
+  }
+  return with_precision(store, compute, fn);
+}
+
 template <class Fn>
 int with_field(int field, Fn&& fn) {
   switch (field) {
@@ -317,49 +376,59 @@ using namespace xlbh;
 
 extern "C" {
 
-int xlbh_stream(int lattice, int store, const void* f_in, void* f_out, int nx, int ny, int nz,
+int xlbh_stream(int lattice, int store, const void* f_in, void* f_out, int nx, int ny, int nz, int64_t plane,
                 void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(valid_store(store), "unknown store format");
+  XLBH_REQUIRE(store != XLBH_I16_BLOCK, "stream: the block format's segment exponents do not move with the "
+                                        "values; stream a decoded field or use the fused step");
   XLBH_REQUIRE(f_in && f_out && f_in != f_out, "stream needs distinct non-null fields");
   const Sweep sw = make_sweep(nx, ny, nz);
-  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  uint64_t N;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, N)) return e;
   hipStream_t s = (hipStream_t)stream;
-  const bool two = (store != XLBH_F32 && store != XLBH_F64);
   return with_lattice(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
-    if (store == XLBH_F64)
-      hipLaunchKernelGGL((k_stream<LAT, double>), sweep_grid(sw), dim3(256), 0, s, (const double*)f_in,
-                         (double*)f_out, sw, N);
-    else if (two)
-      hipLaunchKernelGGL((k_stream<LAT, uint16_t>), sweep_grid(sw), dim3(256), 0, s, (const uint16_t*)f_in,
-                         (uint16_t*)f_out, sw, N);
-    else
-      hipLaunchKernelGGL((k_stream<LAT, float>), sweep_grid(sw), dim3(256), 0, s, (const float*)f_in,
-                         (float*)f_out, sw, N);
-    XLBH_LAUNCH_CHECK();
-    return 0;
+    auto go = [&](auto r) -> int {
+      using R = decltype(r);
+      // the step's occupancy: 4- and 8-byte populations capped at 4 waves per SIMD
+      const unsigned lds = sizeof(R) >= 4 ? kStepLdsCap : 0u;
+      const bool wide = (uint64_t)Lat<LAT>::Q * N * sizeof(R) > 0xffffffffull;
+      if (wide)
+        hipLaunchKernelGGL((k_stream<LAT, R, true>), sweep_grid(sw), dim3(256), lds, s, (const R*)f_in, (R*)f_out,
+                           sw, N);
+      else
+        hipLaunchKernelGGL((k_stream<LAT, R, false>), sweep_grid(sw), dim3(256), lds, s, (const R*)f_in, (R*)f_out,
+                           sw, N);
+      XLBH_LAUNCH_CHECK();
+      return 0;
+    };
+    if (store == XLBH_F64) return go(double{});
+    if (store == XLBH_F32) return go(float{});
+    return go(uint16_t{});
   });
 }
 
 int xlbh_equilibrium(int lattice, int store, int compute, int field, const void* rho, const void* u,
-                     void* f, int nx, int ny, int nz, void* stream) {
+                     void* f, int nx, int ny, int nz, int64_t plane, void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(rho && u && f, "equilibrium needs rho, u and f");
   const Sweep sw = make_sweep(nx, ny, nz);
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   hipStream_t s = (hipStream_t)stream;
   return with_lattice(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
-    return with_precision(store, compute, [&](auto S, auto c) -> int {
+    return with_precision_blk(store, compute, [&](auto S, auto c) -> int {
       constexpr int FMT = decltype(S)::value;
       using C = decltype(c);
       return with_field(field, [&](auto fr) -> int {
         using FR = decltype(fr);
         hipLaunchKernelGGL((k_equilibrium<LAT, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s, (const FR*)rho,
-                           (const FR*)u, (typename Fmt<FMT>::raw*)f, sw, N);
+                           (const FR*)u, (typename Fmt<FMT>::raw*)f, sw, N, P);
         XLBH_LAUNCH_CHECK();
         return 0;
       });
@@ -368,7 +437,7 @@ int xlbh_equilibrium(int lattice, int store, int compute, int field, const void*
 }
 
 int xlbh_moments(int lattice, int store, int compute, int field, int which, const void* f, void* rho,
-                 void* u, void* pi, int nx, int ny, int nz, void* stream) {
+                 void* u, void* pi, int nx, int ny, int nz, int64_t plane, void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(which >= 0 && which <= 3, "moments: which must be 0..3");
@@ -378,16 +447,18 @@ int xlbh_moments(int lattice, int store, int compute, int field, int which, cons
   XLBH_REQUIRE(which != 3 || pi, "moments: pi is null");
   const Sweep sw = make_sweep(nx, ny, nz);
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   hipStream_t s = (hipStream_t)stream;
   return with_lattice(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
-    return with_precision(store, compute, [&](auto S, auto c) -> int {
+    return with_precision_blk(store, compute, [&](auto S, auto c) -> int {
       constexpr int FMT = decltype(S)::value;
       using C = decltype(c);
       return with_field(field, [&](auto fr) -> int {
         using FR = decltype(fr);
         hipLaunchKernelGGL((k_moments<LAT, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
-                           (const typename Fmt<FMT>::raw*)f, (FR*)rho, (FR*)u, (FR*)pi, which, sw, N);
+                           (const typename Fmt<FMT>::raw*)f, (FR*)rho, (FR*)u, (FR*)pi, which, sw, N, P);
         XLBH_LAUNCH_CHECK();
         return 0;
       });
@@ -398,10 +469,15 @@ int xlbh_moments(int lattice, int store, int compute, int field, int which, cons
 int xlbh_collide(int lattice, int collision, int store, int compute, int field, const void* f,
                  const void* feq, void* fout, const void* rho, const void* u, double omega,
                  double collision_param, int force_mode,
-                 const double* force_uniform, const void* force_field, int nx, int ny, int nz, void* stream) {
+                 const double* force_uniform, const void* force_field, int nx, int ny, int nz, int64_t plane,
+                 void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(f && feq && fout, "collide needs f, feq and fout");
+  XLBH_REQUIRE(store != XLBH_I16_BLOCK, "collide: the block format is written by the fused step and the "
+                                        "equilibrium operator only");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC || collision == XLBH_SMAGORINSKY_BGK,
                "unknown collision");
   XLBH_REQUIRE(!(collision == XLBH_KBC && lattice == XLBH_D3Q19),
@@ -429,15 +505,15 @@ int xlbh_collide(int lattice, int collision, int store, int compute, int field, 
         if (collision == XLBH_BGK)
           hipLaunchKernelGGL((k_collide<LAT, XLBH_BGK, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
                              (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega, cp,
-                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
+                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N, P);
         else if (collision == XLBH_SMAGORINSKY_BGK)
           hipLaunchKernelGGL((k_collide<LAT, XLBH_SMAGORINSKY_BGK, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
                              (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega, cp,
-                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
+                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N, P);
         else if constexpr (LAT != XLBH_D3Q19)
           hipLaunchKernelGGL((k_collide<LAT, XLBH_KBC, FMT, C, FR>), sweep_grid(sw), dim3(256), 0, s,
                              (const R*)f, (const R*)feq, (R*)fout, (const FR*)rho, (const FR*)u, (C)omega, cp,
-                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N);
+                             force_mode, fu[0], fu[1], fu[2], (const C*)force_field, sw, N, P);
         XLBH_LAUNCH_CHECK();
         return 0;
       });
@@ -447,10 +523,14 @@ int xlbh_collide(int lattice, int collision, int store, int compute, int field, 
 
 int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
                   const void* f_pre, void* f_post, const uint8_t* bc_mask, const uint32_t* missing_bits,
-                  const void* aux, int nx, int ny, int nz, void* stream) {
+                  const void* aux, int nx, int ny, int nz, int64_t plane, void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(f_pre && f_post && bc_mask, "apply_bc needs f_pre, f_post and bc_mask");
+  XLBH_REQUIRE(store != XLBH_I16_BLOCK, "apply_bc: the block format is written by the fused step and the "
+                                        "equilibrium operator only");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   XLBH_REQUIRE(kind >= XLBH_BC_NONE && kind <= XLBH_BC_GRADS_APPROXIMATION, "unknown BC kind");
   XLBH_REQUIRE(kind != XLBH_BC_GRADS_APPROXIMATION || lattice != XLBH_D2Q9,
                "GradsApproximationBC is 3-D only (bc_grads_approximation.py:75-77)");
@@ -464,7 +544,6 @@ int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id, cons
   xlbh_bc_desc d{};
   if (desc) d = *desc;
   const Sweep sw = make_sweep(nx, ny, nz);
-  const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
   hipStream_t s = (hipStream_t)stream;
   return with_lattice(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
@@ -473,7 +552,7 @@ int xlbh_apply_bc(int lattice, int store, int compute, int kind, int bc_id, cons
       using C = decltype(c);
       using R = typename Fmt<FMT>::raw;
       hipLaunchKernelGGL((k_apply_bc<LAT, FMT, C>), sweep_grid(sw), dim3(256), 0, s, kind, bc_id, d,
-                         (const R*)f_pre, (R*)f_post, bc_mask, missing_bits, (const C*)aux, sw, N);
+                         (const R*)f_pre, (R*)f_post, bc_mask, missing_bits, (const C*)aux, sw, P);
       XLBH_LAUNCH_CHECK();
       return 0;
     });

@@ -50,21 +50,36 @@ __global__ __launch_bounds__(256) void k_final_sum(const double* __restrict__ pa
 // One boundary cell's momentum exchange, added to m: the post-collision populations f_pc, the
 // re-pulled post-stream populations with the BC's streaming functional applied (f_ps), and the sum
 // over missing l of c_{opp l} (f_pc[opp l] + f_ps[l]) (momentum_transfer.py:137-160).
+// Population l of cell (x, y, z) of field fin (plane stride P); the block format looks up the
+// cell's segment exponent in e.
+template <int LAT, int FMT, class C>
+__device__ __forceinline__ C ld_cell(const typename Fmt<FMT>::raw* __restrict__ fin, uint64_t P, const int32_t* e,
+                                     const Sweep& sw, int l, int x, int y, int z) {
+  constexpr LatTables T = make_tables(LAT);
+  const uint32_t cell = ((uint32_t)x * sw.ny + y) * sw.nz + z;
+  if constexpr (FMT == XLBH_I16_BLOCK)
+    return (C)__builtin_fmaf((float)fin[(size_t)l * P + cell], blk_dscale((float)T.w[l], e[blk_seg(sw, x, y, z)]),
+                             (float)T.w[l]);
+  else
+    return ld_pop<FMT, C>(fin + (size_t)l * P + cell, T.w[l]);
+}
+
 template <int LAT, int FMT, class C>
 __device__ __forceinline__ void momentum_cell(const typename Fmt<FMT>::raw* __restrict__ fin, uint32_t mm,
-                                              const C* aux, int kind, const xlbh_bc_desc& desc, int nx, int ny,
-                                              int nz, uint32_t N, uint32_t cell, double (&m)[kRedWidth]) {
+                                              const C* aux, int kind, const xlbh_bc_desc& desc, const Sweep& sw,
+                                              uint64_t P, const int32_t* e, uint32_t cell, double (&m)[kRedWidth]) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
+  const int nx = sw.nx, ny = sw.ny, nz = sw.nz;
   const int z = cell % nz;
   const int y = (cell / nz) % ny;
   const int x = cell / ((uint32_t)nz * ny);
   C fpc[Q], fps[Q];
 #pragma unroll
   for (int l = 0; l < Q; ++l) {
-    fpc[l] = ld_pop<FMT, C>(fin + (size_t)l * N + cell, T.w[l]);
+    fpc[l] = ld_cell<LAT, FMT, C>(fin, P, e, sw, l, x, y, z);
     const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
-    fps[l] = ld_pop<FMT, C>(fin + (size_t)l * N + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
+    fps[l] = ld_cell<LAT, FMT, C>(fin, P, e, sw, l, sx, sy, sz);
   }
   if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
 #pragma unroll
@@ -111,8 +126,8 @@ template <int LAT, int FMT, class C>
 __global__ __launch_bounds__(256) void k_momentum(const typename Fmt<FMT>::raw* __restrict__ fin,
                                                   const uint8_t* __restrict__ bc_mask,
                                                   const uint32_t* __restrict__ missing, const C* aux, int kind,
-                                                  int id, xlbh_bc_desc desc, int nx, int ny, int nz, uint32_t N,
-                                                  double* __restrict__ part) {
+                                                  int id, xlbh_bc_desc desc, Sweep sw, uint32_t N, uint64_t P,
+                                                  const int32_t* e, double* __restrict__ part) {
   const uint32_t base = blockIdx.x * kRedCells + threadIdx.x;
   uint8_t ids[kRedRun];
 #pragma unroll
@@ -125,7 +140,7 @@ __global__ __launch_bounds__(256) void k_momentum(const typename Fmt<FMT>::raw* 
     const uint32_t cell = base + 256u * k;
     if (ids[k] != id) continue;
     const uint32_t mm = missing[cell];
-    if (!bit(mm, 0)) momentum_cell<LAT, FMT, C>(fin, mm, aux, kind, desc, nx, ny, nz, N, cell, m);
+    if (!bit(mm, 0)) momentum_cell<LAT, FMT, C>(fin, mm, aux, kind, desc, sw, P, e, cell, m);
   }
   block_sum<kRedWidth>(m, part + (size_t)blockIdx.x * kRedWidth);
 }
@@ -133,16 +148,25 @@ __global__ __launch_bounds__(256) void k_momentum(const typename Fmt<FMT>::raw* 
 // sum rho and sum c f over non-solid cells, accumulated in FP64 from the stored populations.
 template <int LAT, int FMT>
 __global__ __launch_bounds__(256) void k_mass_momentum(const typename Fmt<FMT>::raw* __restrict__ fin,
-                                                       const uint8_t* __restrict__ bc_mask, uint32_t N,
-                                                       double* __restrict__ part) {
+                                                       const uint8_t* __restrict__ bc_mask, Sweep sw, uint32_t N,
+                                                       uint64_t P, const int32_t* e, double* __restrict__ part) {
   constexpr LatTables T = make_tables(LAT);
   double v[kRedWidth] = {0, 0, 0, 0};
   for (int k = 0; k < kRedRun; ++k) {
     const uint32_t cell = blockIdx.x * kRedCells + threadIdx.x + 256u * k;
     if (cell < N && (bc_mask == nullptr || bc_mask[cell] != 255)) {
+      int kb = 0;
+      if constexpr (FMT == XLBH_I16_BLOCK) {
+        const int z = cell % sw.nz, y = (cell / sw.nz) % sw.ny, x = cell / ((uint32_t)sw.nz * sw.ny);
+        kb = e[blk_seg(sw, x, y, z)];
+      }
 #pragma unroll
       for (int l = 0; l < T.q; ++l) {
-        const double f = ld_pop<FMT, double>(fin + (size_t)l * N + cell, T.w[l]);
+        double f;
+        if constexpr (FMT == XLBH_I16_BLOCK)
+          f = (double)__builtin_fmaf((float)fin[(size_t)l * P + cell], blk_dscale((float)T.w[l], kb), (float)T.w[l]);
+        else
+          f = ld_pop<FMT, double>(fin + (size_t)l * P + cell, T.w[l]);
         v[0] += f;
 #pragma unroll
         for (int d = 0; d < T.d; ++d) v[1 + d] += (double)T.c[d][l] * f;
@@ -177,10 +201,13 @@ int64_t xlbh_reduce_workspace_bytes(int64_t n_cells) {
 
 int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
                            const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
-                           double* out, void* workspace, int nx, int ny, int nz, void* stream) {
+                           double* out, void* workspace, int nx, int ny, int nz, int64_t plane, void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(f && bc_mask && missing_bits && out && workspace, "momentum_transfer: null argument");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
+  const Sweep sw = make_sweep(nx, ny, nz);
   XLBH_REQUIRE(bc_id >= 1 && bc_id <= 254, "momentum_transfer: bc_id must be 1..254");
   XLBH_REQUIRE(!(kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) || aux,
                "momentum_transfer: Zou-He / regularized BCs need the aux table");
@@ -195,9 +222,11 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
     auto go = [&](auto S, auto c) -> int {
       constexpr int FMT = decltype(S)::value;
       using C = decltype(c);
+      const int32_t* ex = reinterpret_cast<const int32_t*>(static_cast<const char*>(f) +
+                                                           blk_exps_offset(Lat<LAT>::Q, P));
       hipLaunchKernelGGL((k_momentum<LAT, FMT, C>), dim3(nb), dim3(256), 0, s,
                          (const typename Fmt<FMT>::raw*)f, bc_mask, missing_bits, (const C*)aux, kind, bc_id, d,
-                         nx, ny, nz, N, part);
+                         sw, N, P, ex, part);
       XLBH_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, part, nb, Lat<LAT>::D, out);
       XLBH_LAUNCH_CHECK();
@@ -210,6 +239,7 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
         case XLBH_BF16: return go(std::integral_constant<int, XLBH_BF16>{}, float{});
         case XLBH_F16_SHIFTED: return go(std::integral_constant<int, XLBH_F16_SHIFTED>{}, float{});
         case XLBH_BF16_SHIFTED: return go(std::integral_constant<int, XLBH_BF16_SHIFTED>{}, float{});
+        case XLBH_I16_BLOCK: return go(std::integral_constant<int, XLBH_I16_BLOCK>{}, float{});
       }
     } else if (compute == XLBH_COMPUTE_F64) {
       switch (store) {
@@ -224,11 +254,14 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
 }
 
 int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask, double* out, void* workspace,
-                       int nx, int ny, int nz, void* stream) {
+                       int nx, int ny, int nz, int64_t plane, void* stream) {
   clear_error();
   if (int e = check_grid(lattice, nx, ny, nz)) return e;
   XLBH_REQUIRE(f && out && workspace, "mass_momentum: null argument");
   XLBH_REQUIRE(valid_store(store), "unknown store format");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
+  const Sweep sw = make_sweep(nx, ny, nz);
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
   const int nb = (int)red_blocks(N);
   double* part = static_cast<double*>(workspace);
@@ -237,8 +270,10 @@ int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_
     constexpr int LAT = decltype(L)::value;
     auto go = [&](auto S) -> int {
       constexpr int FMT = decltype(S)::value;
+      const int32_t* ex = reinterpret_cast<const int32_t*>(static_cast<const char*>(f) +
+                                                           blk_exps_offset(Lat<LAT>::Q, P));
       hipLaunchKernelGGL((k_mass_momentum<LAT, FMT>), dim3(nb), dim3(256), 0, s,
-                         (const typename Fmt<FMT>::raw*)f, bc_mask, N, part);
+                         (const typename Fmt<FMT>::raw*)f, bc_mask, sw, N, P, ex, part);
       XLBH_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, s, part, nb, 1 + Lat<LAT>::D, out);
       XLBH_LAUNCH_CHECK();
@@ -251,6 +286,7 @@ int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_
       case XLBH_BF16: return go(std::integral_constant<int, XLBH_BF16>{});
       case XLBH_F16_SHIFTED: return go(std::integral_constant<int, XLBH_F16_SHIFTED>{});
       case XLBH_BF16_SHIFTED: return go(std::integral_constant<int, XLBH_BF16_SHIFTED>{});
+      case XLBH_I16_BLOCK: return go(std::integral_constant<int, XLBH_I16_BLOCK>{});
     }
     set_error("unknown store format");
     return 1;

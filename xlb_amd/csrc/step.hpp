@@ -35,22 +35,54 @@ struct StepParams {
   double cparam;  // Smagorinsky constant (XLBH_SMAGORINSKY_BGK)
   Sweep sw;
   uint32_t N;
+  uint64_t plane;        // population plane stride (elements)
+  const int32_t* e_in;   // XLBH_I16_BLOCK: segment exponents of f_in and f_out
+  int32_t* e_out;
 };
 
 // Population field addressing.  Plane l of a field starts at base + l * plane_bytes (uniform,
 // held in scalar registers); a cell is reached with an unsigned per-lane offset.  With
 // O = uint32_t (every plane below 4 GiB) the compiler emits the scalar-base + 32-bit vector-offset
 // form of global loads/stores; O = uint64_t serves larger planes.
+#ifndef XLBH_ONE_BASE
+#define XLBH_ONE_BASE 1
+#endif
 template <class R, class O>
 struct Planes {
   const char* base;
   size_t plane_bytes;
   __device__ __forceinline__ const R* at(int l, O cell) const {
+#if XLBH_ONE_BASE
+    if constexpr (sizeof(O) == 4)
+      return reinterpret_cast<const R*>(base + (uint32_t)((uint32_t)l * (uint32_t)plane_bytes + cell * (O)sizeof(R)));
+#endif
     return reinterpret_cast<const R*>(base + (size_t)l * plane_bytes + cell * (O)sizeof(R));
   }
   __device__ __forceinline__ R* at_w(int l, O cell) const { return const_cast<R*>(at(l, cell)); }
 };
 
+// Reads of stored populations other than the pulls: the own cell's pre-stream values and the
+// extrapolation outflow's source cell.  For the block format, `k_own` is the exponent of the
+// caller's segment and other cells look theirs up.
+template <int FMT, class C, class O>
+struct PopIn {
+  Planes<typename Fmt<FMT>::raw, O> pl;
+  const int32_t* e;
+  Sweep sw;
+  int k_own;
+  __device__ __forceinline__ C own(int l, O cell, double w) const {
+    if constexpr (FMT == XLBH_I16_BLOCK)
+      return __builtin_fmaf((float)*pl.at(l, cell), blk_dscale((float)w, k_own), (float)w);
+    else
+      return ld_pop<FMT, C>(pl.at(l, cell), w);
+  }
+  __device__ __forceinline__ C at(int l, O cell, int x, int y, int z, double w) const {
+    if constexpr (FMT == XLBH_I16_BLOCK)
+      return __builtin_fmaf((float)*pl.at(l, cell), blk_dscale((float)w, e[blk_seg(sw, x, y, z)]), (float)w);
+    else
+      return ld_pop<FMT, C>(pl.at(l, cell), w);
+  }
+};
 
 // Pull-source offsets x - c for c in {-1, 0, +1} per axis, with periodic wrap (stream.py:64-81).
 template <class O>
@@ -80,7 +112,7 @@ struct PullOffsets {
 // extrapolation-outflow update (nse_stepper.py:360-378).
 template <int LAT, int COLL, int FMT, class C, class O>
 __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, int z, const StepParams& p,
-                                            const Planes<typename Fmt<FMT>::raw, O>& fin) {
+                                            const PopIn<FMT, C, O>& fin) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
@@ -105,17 +137,17 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
     } else if (kind == XLBH_BC_HALFWAY_BB || kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
 #pragma unroll
       for (int l = 0; l < Q; ++l)
-        if (bit(mm, l)) f[l] = ld_pop<FMT, C>(fin.at(T.opp[l], cell), T.w[T.opp[l]]);
+        if (bit(mm, l)) f[l] = fin.own(T.opp[l], cell, T.w[T.opp[l]]);
     } else if (kind == XLBH_BC_DO_NOTHING) {
 #pragma unroll
-      for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fin.at(l, cell), T.w[l]);
+      for (int l = 0; l < Q; ++l) f[l] = fin.own(l, cell, T.w[l]);
     } else if (kind >= XLBH_BC_ZOUHE_VELOCITY && kind <= XLBH_BC_REGULARIZED_PRESSURE) {
       const C pv = p.aux != nullptr ? static_cast<const C*>(p.aux)[cell] : C(0);
       const bool vel = (kind == XLBH_BC_ZOUHE_VELOCITY || kind == XLBH_BC_REGULARIZED_VELOCITY);
       const bool reg = (kind >= XLBH_BC_REGULARIZED_VELOCITY);
       zouhe_cell<LAT>(f, mm, pv, vel, reg);
     } else if (kind == XLBH_BC_GRADS_APPROXIMATION) {
-      grads_cell<LAT>(f, mm, [&](int l) { return ld_pop<FMT, C>(fin.at(l, cell), T.w[l]); });
+      grads_cell<LAT>(f, mm, [&](int l) { return fin.own(l, cell, T.w[l]); });
     }
   }
 
@@ -157,29 +189,74 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
         const int sy = wrap(y - (T.c[1][l] + n[1]), ny);
         const int sz = wrap(z - (T.c[2][l] + n[2]), nz);
         const O src = ((O)sx * ny + sy) * nz + sz;
-        const C faux = ld_pop<FMT, C>(fin.at(l, src), T.w[l]);
-        const C fps = ld_pop<FMT, C>(fin.at(T.opp[l], cell), T.w[T.opp[l]]);
+        const C faux = fin.at(l, src, sx, sy, sz, T.w[l]);
+        const C fps = fin.own(T.opp[l], cell, T.w[T.opp[l]]);
         f[T.opp[l]] = (C(1) - cs) * fps + cs * faux;
       }
     }
   }
 }
 
-// One thread per cell: Q pulls, the update, Q stores.
-template <int LAT, int COLL, int FMT, class C, bool WIDE>
+// Exponents of the Q pulls of a block-format cell: k of each pull's source segment.  (xr, yr) is
+// the wave's row (3-D: x, y; 2-D: x, with yr unused), uniform across the wave; `fast` the lane's
+// coordinate on the fast axis, `fb` the block's fast segment.  Pulls along the fast axis cross
+// into the neighbouring segment only at segment edges, and only when the row has more than one
+// segment (MSEG).
+template <int LAT, bool MSEG>
+__device__ __forceinline__ void blk_pull_exps(const StepParams& p, int xr, int yr, int fast, int fb,
+                                              int (&kk)[Lat<LAT>::Q]) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr bool three = T.d == 3;
+  const Sweep& sw = p.sw;
+  const int S = (int)blk_segs_fast(sw);
+  const int nfast = three ? sw.nz : sw.ny;
+  const int fb_lo = fb == 0 ? S - 1 : fb - 1, fb_hi = fb == S - 1 ? 0 : fb + 1;
+  const bool lo_cross = MSEG && ((fast == 0 ? nfast - 1 : fast - 1) >> sw.lbz) != fb;
+  const bool hi_cross = MSEG && ((fast == nfast - 1 ? 0 : fast + 1) >> sw.lbz) != fb;
+#pragma unroll
+  for (int l = 0; l < T.q; ++l) {
+    const int cf = three ? T.c[2][l] : T.c[1][l];
+    uint32_t row;
+    if constexpr (three) {
+      const int xs = wrap1(xr - T.c[0][l], sw.nx), ys = wrap1(yr - T.c[1][l], sw.ny);
+      row = ((uint32_t)xs * sw.ny + ys) * S;
+    } else {
+      row = (uint32_t)wrap1(xr - T.c[0][l], sw.nx) * S;
+    }
+    int k = p.e_in[row + fb];
+    if constexpr (MSEG) {
+      if (cf == 1) k = lo_cross ? p.e_in[row + fb_lo] : k;
+      if (cf == -1) k = hi_cross ? p.e_in[row + fb_hi] : k;
+    }
+    kk[l] = k;
+  }
+}
+
+// One thread per cell: Q pulls, the update, Q stores.  For the block format every lane of the
+// block stays to the segment reduction: lanes outside the grid read a cell of their own row and
+// write nothing, and solid cells write nothing.
+template <int LAT, int COLL, int FMT, class C, bool WIDE, bool MSEG>
 __device__ __forceinline__ void step_body(const StepParams& p) {
   constexpr LatTables T = make_tables(LAT);
   constexpr int Q = T.q;
+  constexpr bool BLK = FMT == XLBH_I16_BLOCK;
   using R = typename Fmt<FMT>::raw;
   using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
 
   int x, y, z;
-  if (!sweep_cell(p.sw, x, y, z)) return;
+  const bool live = sweep_cell(p.sw, x, y, z);
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
+  if constexpr (!BLK) {
+    if (!live) return;
+  } else {
+    x = min(x, nx - 1);
+    y = min(y, ny - 1);
+    z = min(z, nz - 1);
+  }
   const O cell = ((O)x * ny + y) * nz + z;
-  const size_t plane_bytes = (size_t)p.N * sizeof(R);
-  const Planes<R, O> fin{static_cast<const char*>(p.f_in), plane_bytes};
+  const size_t plane_bytes = (size_t)p.plane * sizeof(R);
   const Planes<R, O> fout{static_cast<const char*>(p.f_out), plane_bytes};
+  PopIn<FMT, C, O> fin{{static_cast<const char*>(p.f_in), plane_bytes}, p.e_in, p.sw, 0};
   const PullOffsets<O> po(x, y, z, nx, ny, nz);
 
   // The boundary id is loaded first and the pulls right after it, with no wait in between (every
@@ -190,13 +267,51 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
   int id = 0;
   if (p.bc_mask != nullptr) id = __builtin_nontemporal_load(p.bc_mask + cell);
   C f[Q];
+  [[maybe_unused]] int xr = 0, yr = 0, fb = 0;
+  if constexpr (BLK) {
+    // The exponent loads (scalar: nothing before them writes memory) go out first, the pulls
+    // right behind them; the scales are formed once both are in flight.
+    constexpr bool three = T.d == 3;
+    xr = __builtin_amdgcn_readfirstlane(x);
+    yr = three ? __builtin_amdgcn_readfirstlane(y) : 0;
+    fb = blockIdx.x;
+    int kk[Q];
+    blk_pull_exps<LAT, MSEG>(p, xr, yr, three ? z : y, fb, kk);
+    fin.k_own = kk[0];  // the rest direction pulls from the cell itself
+    int m[Q];
 #pragma unroll
-  for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fin.at(l, po.template src<LAT>(l)), T.w[l]);
+    for (int l = 0; l < Q; ++l) m[l] = ld_stream(fin.pl.at(l, po.template src<LAT>(l)));
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int l = 0; l < Q; ++l) f[l] = __builtin_fmaf((float)m[l], blk_dscale((float)T.w[l], kk[l]), (float)T.w[l]);
+  } else {
+#pragma unroll
+    for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fin.pl.at(l, po.template src<LAT>(l)), T.w[l]);
+  }
   asm volatile("" ::: "memory");
-  if (id == 255) return;  // solid: no update (nse_stepper.py:356-358)
-  cell_update<LAT, COLL, FMT, C, O>(f, id, cell, x, y, z, p, fin);
+  if constexpr (!BLK) {
+    if (id == 255) return;  // solid: no update (nse_stepper.py:356-358)
+    cell_update<LAT, COLL, FMT, C, O>(f, id, cell, x, y, z, p, fin);
 #pragma unroll
-  for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout.at_w(l, cell), f[l], T.w[l]);
+    for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout.at_w(l, cell), f[l], T.w[l]);
+  } else {
+    const bool write = live && id != 255;
+    if (write) cell_update<LAT, COLL, FMT, C, O>(f, id, cell, x, y, z, p, fin);
+    const int k = blk_segment_k(write ? blk_cell_max<LAT>(f) : 0.0f, p.sw.lbz);
+    if ((threadIdx.x & ((1u << p.sw.lbz) - 1)) == 0) p.e_out[blk_row_seg(p.sw, xr, yr, fb)] = k;
+    if (write) {
+      if (k != kBlkNonFinite) {
+        const float nk = -blk_pow2(k);
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+          st_stream(fout.at_w(l, cell),
+                    blk_round(__builtin_fmaf(f[l], blk_escale((float)(1.0 / T.w[l]), k), nk)));
+      } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) st_stream(fout.at_w(l, cell), (int16_t)32767);
+      }
+    }
+  }
 }
 
 // Occupancy.  Measured on the 256^3 D3Q19 sweep: 4-byte populations stream best at 4 waves per
@@ -209,13 +324,25 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
 constexpr unsigned kStepLdsCap = 36 * 1024;
 
 template <int LAT, int COLL, int FMT, class C, bool WIDE>
+__device__ __forceinline__ void step_dispatch(const StepParams& p) {
+  if constexpr (FMT == XLBH_I16_BLOCK) {
+    if (blk_segs_fast(p.sw) > 1)
+      step_body<LAT, COLL, FMT, C, WIDE, true>(p);
+    else
+      step_body<LAT, COLL, FMT, C, WIDE, false>(p);
+  } else {
+    step_body<LAT, COLL, FMT, C, WIDE, false>(p);
+  }
+}
+
+template <int LAT, int COLL, int FMT, class C, bool WIDE>
 __global__ __launch_bounds__(256) void k_step(StepParams p) {
-  step_body<LAT, COLL, FMT, C, WIDE>(p);
+  step_dispatch<LAT, COLL, FMT, C, WIDE>(p);
 }
 
 template <int LAT, int COLL, int FMT, class C, bool WIDE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_step_w6(StepParams p) {
-  step_body<LAT, COLL, FMT, C, WIDE>(p);
+  step_dispatch<LAT, COLL, FMT, C, WIDE>(p);
 }
 
 
@@ -232,6 +359,7 @@ struct StepCall {
   const double* force_uniform;
   const void* force_field;
   int nx, ny, nz;
+  uint64_t plane;  // population plane stride (elements, >= N)
   double omega;
   double cparam;  // Smagorinsky constant (XLBH_SMAGORINSKY_BGK)
   hipStream_t stream;
@@ -253,7 +381,16 @@ int launch_step_t(const StepCall& c) {
   p.cparam = c.cparam;
   p.sw = make_sweep(c.nx, c.ny, c.nz);
   p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
-  const bool wide = (uint64_t)p.N * sizeof(typename Fmt<FMT>::raw) > 0xffffffffull;
+  p.plane = c.plane;
+  p.e_in = nullptr;
+  p.e_out = nullptr;
+  if constexpr (FMT == XLBH_I16_BLOCK) {
+    const size_t off = blk_exps_offset(Lat<LAT>::Q, c.plane);
+    p.e_in = reinterpret_cast<const int32_t*>(static_cast<const char*>(c.f_in) + off);
+    p.e_out = reinterpret_cast<int32_t*>(static_cast<char*>(c.f_out) + off);
+  }
+  // 32-bit per-lane byte offsets reach every plane of the field from its base
+  const bool wide = (uint64_t)Lat<LAT>::Q * p.plane * sizeof(typename Fmt<FMT>::raw) > 0xffffffffull;
   const unsigned lds = sizeof(typename Fmt<FMT>::raw) >= 4 ? kStepLdsCap : 0u;
   constexpr bool w6 = LAT == XLBH_D3Q19 && sizeof(typename Fmt<FMT>::raw) == 2 && std::is_same<C, float>::value;
   if (wide)
@@ -283,6 +420,7 @@ int launch_step_lat(const StepCall& c) {
       case XLBH_BF16: return launch_step_t<LAT, COLL, XLBH_BF16, float>(c);                     \
       case XLBH_F16_SHIFTED: return launch_step_t<LAT, COLL, XLBH_F16_SHIFTED, float>(c);       \
       case XLBH_BF16_SHIFTED: return launch_step_t<LAT, COLL, XLBH_BF16_SHIFTED, float>(c);     \
+      case XLBH_I16_BLOCK: return launch_step_t<LAT, COLL, XLBH_I16_BLOCK, float>(c);           \
       default: break;                                                                           \
     }                                                                                           \
   } else if (c.compute == XLBH_COMPUTE_F64) {                                                   \

@@ -17,8 +17,9 @@ def create_nse_fields(grid_shape: Tuple[int, int, int] = None, grid=None, veloci
         if grid_shape is None:
             raise ValueError("grid_shape must be provided when grid is None")
         grid = grid_factory(grid_shape, compute_backend=compute_backend)
-    f_0 = grid.create_field(cardinality=velocity_set.q, dtype=precision_policy.store_precision)
-    f_1 = grid.create_field(cardinality=velocity_set.q, dtype=precision_policy.store_precision)
+    # populations: plane-padded layout (same indexing as a contiguous field; see create_field)
+    f_0 = grid.create_field(cardinality=velocity_set.q, dtype=precision_policy.store_precision, padded=True)
+    f_1 = grid.create_field(cardinality=velocity_set.q, dtype=precision_policy.store_precision, padded=True)
     missing_mask = grid.create_field(cardinality=velocity_set.q, dtype=Precision.BOOL)
     bc_mask = grid.create_field(cardinality=1, dtype=Precision.UINT8)
     return grid, f_0, f_1, missing_mask, bc_mask

@@ -23,7 +23,7 @@ import torch
 from xlb_amd import _lib
 from xlb_amd.compute_backend import ComputeBackend
 from xlb_amd.operator.boundary_masker.packing import cell_coordinates, cells_with_id, packed_missing
-from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.operator import Operator, expect_field, field_extents, shared_layout
 
 
 class ImplementationStep(Enum):
@@ -117,7 +117,7 @@ class BoundaryCondition(Operator):
         if cells.numel() > 0:
             index = cell_coordinates(cells, (nx, ny, nz), self.velocity_set.d)
             vals = torch.from_numpy(self._aux_values(index))
-            if not self.precision_policy.shifted:
+            if not (self.precision_policy.shifted or self.precision_policy.block):
                 vals = vals.to(self.store_dtype)  # the reference stores it in the store dtype
             vals = vals.to(self.compute_dtype).to(bc_mask.device)
             _lib.check(
@@ -135,8 +135,10 @@ class BoundaryCondition(Operator):
         _lib.require_cuda(f_pre, f_post, bc_mask, missing_mask)
         nx, ny, nz = field_extents(f_post)
         q, ext = self.velocity_set.q, (nx, ny, nz)
-        expect_field(f_post, q, ext, "f_post")
-        expect_field(f_pre, q, ext, "f_pre", f_post.dtype)
+        for t, name in ((f_post, "f_post"), (f_pre, "f_pre")):
+            if tuple(t.shape) != (q,) + ext or t.dtype != f_post.dtype:
+                raise ValueError(f"{name} must be a {(q,) + ext} field of dtype {f_post.dtype}")
+        plane, (pre_, post_) = shared_layout([f_pre, f_post])
         expect_field(bc_mask, 1, ext, "bc_mask", torch.uint8)
         expect_field(missing_mask, q, ext, "missing_mask", torch.bool)
         bits = packed_missing(missing_mask, self.lattice_id)
@@ -149,11 +151,13 @@ class BoundaryCondition(Operator):
         _lib.check(
             _lib.lib().xlbh_apply_bc(
                 self.lattice_id, self.format_of(f_post), self.compute_id, self.kind, int(self.id), desc,
-                _lib.ptr(f_pre), _lib.ptr(f_post), _lib.ptr(bc_mask), _lib.ptr(bits), _lib.ptr(aux), nx, ny, nz,
+                _lib.ptr(pre_), _lib.ptr(post_), _lib.ptr(bc_mask), _lib.ptr(bits), _lib.ptr(aux), nx, ny, nz, plane,
                 _lib.stream_of(f_post),
             ),
             self.__class__.__name__,
         )
+        if post_ is not f_post:
+            f_post.copy_(post_)
         _lib.mark_written(f_post)
         return f_post
 

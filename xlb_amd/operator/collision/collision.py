@@ -5,7 +5,7 @@ import numpy as np
 
 from xlb_amd import _lib
 from xlb_amd.compute_backend import ComputeBackend
-from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.operator import Operator, expect_field, field_extents, shared_layout
 from xlb_amd.precision_policy import Precision
 
 
@@ -15,9 +15,10 @@ def _collide(op, collision, f, feq, fout, rho, u, omega, force_mode=_lib.FORCE_N
     nx, ny, nz = field_extents(f)
     vs = op.velocity_set
     ext = (nx, ny, nz)
-    expect_field(f, vs.q, ext, "f")
-    expect_field(feq, vs.q, ext, "feq", f.dtype)
-    expect_field(fout, vs.q, ext, "fout", f.dtype)
+    for t, name in ((f, "f"), (feq, "feq"), (fout, "fout")):
+        if t is None or tuple(t.shape) != (vs.q,) + ext or t.dtype != f.dtype:
+            raise ValueError(f"{name} must be a {(vs.q,) + ext} field of dtype {f.dtype}")
+    plane, (f_, feq_, fout_) = shared_layout([f, feq, fout])
     if collision == _lib.KBC or force_mode != _lib.FORCE_NONE:
         expect_field(rho, 1, ext, "rho")
         expect_field(u, vs.d, ext, "u")
@@ -35,11 +36,13 @@ def _collide(op, collision, f, feq, fout, rho, u, omega, force_mode=_lib.FORCE_N
     _lib.check(
         _lib.lib().xlbh_collide(
             op.lattice_id, collision, op.format_of(f), op.compute_id, field_id,
-            _lib.ptr(f), _lib.ptr(feq), _lib.ptr(fout), _lib.ptr(rho), _lib.ptr(u), float(omega),
-            float(collision_param), force_mode, fu_ptr, _lib.ptr(force_field), nx, ny, nz, _lib.stream_of(f),
+            _lib.ptr(f_), _lib.ptr(feq_), _lib.ptr(fout_), _lib.ptr(rho), _lib.ptr(u), float(omega),
+            float(collision_param), force_mode, fu_ptr, _lib.ptr(force_field), nx, ny, nz, plane, _lib.stream_of(f),
         ),
         type(op).__name__,
     )
+    if fout_ is not fout:
+        fout.copy_(fout_)
     _lib.mark_written(fout)
     return fout
 

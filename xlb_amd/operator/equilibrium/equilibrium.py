@@ -5,7 +5,7 @@ import torch
 
 from xlb_amd import _lib
 from xlb_amd.compute_backend import ComputeBackend
-from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.operator import Operator, expect_field, expect_population, field_extents
 from xlb_amd.precision_policy import Precision
 
 
@@ -24,13 +24,14 @@ class QuadraticEquilibrium(Equilibrium):
         rho = rho.contiguous()
         u = u.contiguous()
         nx, ny, nz = field_extents(f)
-        expect_field(f, self.velocity_set.q, (nx, ny, nz), "f")
+        fmt = self.format_of(f)
+        plane = expect_population(f, self.velocity_set.q, (nx, ny, nz), "f", f.dtype, self.lattice_id, fmt)
         expect_field(rho, 1, (nx, ny, nz), "rho")
         expect_field(u, self.velocity_set.d, (nx, ny, nz), "u")
         _lib.check(
             _lib.lib().xlbh_equilibrium(
-                self.lattice_id, self.format_of(f), self.compute_id, Precision.from_torch(rho.dtype).field_id,
-                _lib.ptr(rho), _lib.ptr(u), _lib.ptr(f), nx, ny, nz, _lib.stream_of(f),
+                self.lattice_id, fmt, self.compute_id, Precision.from_torch(rho.dtype).field_id,
+                _lib.ptr(rho), _lib.ptr(u), _lib.ptr(f), nx, ny, nz, plane, _lib.stream_of(f),
             ),
             "QuadraticEquilibrium",
         )

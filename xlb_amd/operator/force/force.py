@@ -7,7 +7,7 @@ import torch
 from xlb_amd import _lib
 from xlb_amd.compute_backend import ComputeBackend
 from xlb_amd.operator.boundary_masker.packing import packed_missing
-from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.operator import Operator, expect_field, expect_population, field_extents
 
 
 class MomentumTransfer(Operator):
@@ -29,7 +29,8 @@ class MomentumTransfer(Operator):
         _lib.require_cuda(f_0, bc_mask, missing_mask)
         nx, ny, nz = field_extents(f_0)
         q, ext = self.velocity_set.q, (nx, ny, nz)
-        expect_field(f_0, q, ext, "f_0")
+        fmt = self.format_of(f_0)
+        plane = expect_population(f_0, q, ext, "f_0", f_0.dtype, self.lattice_id, fmt)
         expect_field(bc_mask, 1, ext, "bc_mask", torch.uint8)
         expect_field(missing_mask, q, ext, "missing_mask", torch.bool)
         N = nx * ny * nz
@@ -46,9 +47,9 @@ class MomentumTransfer(Operator):
             aux = bc.aux
         _lib.check(
             L.xlbh_momentum_transfer(
-                self.lattice_id, self.format_of(f_0), self.compute_id, bc.kind, int(bc.id), bc.bc_desc(),
+                self.lattice_id, fmt, self.compute_id, bc.kind, int(bc.id), bc.bc_desc(),
                 _lib.ptr(f_0), _lib.ptr(bc_mask), _lib.ptr(bits), _lib.ptr(aux), _lib.ptr(out), _lib.ptr(self._ws),
-                nx, ny, nz, _lib.stream_of(f_0),
+                nx, ny, nz, plane, _lib.stream_of(f_0),
             ),
             "MomentumTransfer",
         )

@@ -3,7 +3,7 @@ zero_moment.py:11-49, first_moment.py:11-67, second_moment.py:13-106)."""
 
 from xlb_amd import _lib
 from xlb_amd.compute_backend import ComputeBackend
-from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.operator import Operator, expect_field, expect_population, field_extents
 from xlb_amd.precision_policy import Precision
 
 
@@ -15,7 +15,8 @@ def _moments(op, which, f, rho, u, pi):
             raise ValueError("rho, u and pi must share one dtype")
     nx, ny, nz = field_extents(f)
     vs = op.velocity_set
-    expect_field(f, vs.q, (nx, ny, nz), "f")
+    fmt = op.format_of(f)
+    plane = expect_population(f, vs.q, (nx, ny, nz), "f", f.dtype, op.lattice_id, fmt)
     if rho is not None:
         expect_field(rho, 1, (nx, ny, nz), "rho")
     if u is not None:
@@ -24,8 +25,8 @@ def _moments(op, which, f, rho, u, pi):
         expect_field(pi, vs.d * (vs.d + 1) // 2, (nx, ny, nz), "pi")
     _lib.check(
         _lib.lib().xlbh_moments(
-            op.lattice_id, op.format_of(f), op.compute_id, Precision.from_torch(ref.dtype).field_id, which,
-            _lib.ptr(f), _lib.ptr(rho), _lib.ptr(u), _lib.ptr(pi), nx, ny, nz, _lib.stream_of(f),
+            op.lattice_id, fmt, op.compute_id, Precision.from_torch(ref.dtype).field_id, which,
+            _lib.ptr(f), _lib.ptr(rho), _lib.ptr(u), _lib.ptr(pi), nx, ny, nz, plane, _lib.stream_of(f),
         ),
         type(op).__name__,
     )

@@ -102,13 +102,73 @@ class Operator:
         return self.precision_policy.compute_id
 
 
+def plane_stride(f):
+    """Elements between consecutive planes of a (card, nx, ny, nz) field laid out in C order within
+    each plane: nx*ny*nz for a contiguous tensor, more for a plane-padded population field
+    (``create_field(..., padded=True)``).  None if the field has another layout."""
+    if f.dim() != 4:
+        return None
+    card, nx, ny, nz = (int(v) for v in f.shape)
+    n = nx * ny * nz
+    want = (None, ny * nz, nz, 1)
+    for d in (1, 2, 3):
+        if f.shape[d] > 1 and f.stride(d) != want[d]:
+            return None
+    if card == 1:
+        return n
+    p = int(f.stride(0))
+    return p if p >= n else None
+
+
 def field_extents(f):
-    """(nx, ny, nz) of a (card, nx, ny, nz) field."""
+    """(nx, ny, nz) of a (card, nx, ny, nz) field (contiguous, or plane-padded)."""
     if f.dim() != 4:
         raise ValueError(f"fields must have shape (cardinality, nx, ny, nz); got {tuple(f.shape)}")
-    if not f.is_contiguous():
-        raise ValueError("fields must be contiguous (C order, z unit-stride)")
+    if plane_stride(f) is None:
+        raise ValueError("fields must be C-ordered (z unit-stride) within each plane")
     return int(f.shape[1]), int(f.shape[2]), int(f.shape[3])
+
+
+def expect_population(t, q, extents, name, dtype, lattice, store):
+    """Check a population field (shape (q, *extents), C order within each plane, on the GPU, of
+    dtype) and that its storage holds every plane (and, for the block format, its segment
+    exponents); returns its plane stride."""
+    if t is None:
+        raise ValueError(f"{name} is required")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on the GPU (device='cuda')")
+    if tuple(t.shape) != (int(q),) + tuple(extents):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}; expected {(int(q),) + tuple(extents)}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} has dtype {t.dtype}; expected {dtype}")
+    plane = plane_stride(t)
+    if plane is None:
+        raise ValueError(f"{name} must be C-ordered within each plane (a contiguous tensor or a plane-padded field)")
+    need = int(_lib.lib().xlbh_population_bytes(lattice, store, *extents, plane))
+    have = t.untyped_storage().nbytes() - t.storage_offset() * t.element_size()
+    if need < 0 or have < need:
+        what = " and its segment exponents" if store == _lib.I16_BLOCK else ""
+        raise ValueError(f"{name}: its storage holds {have} bytes, fewer than the {need} its planes{what} need "
+                         "(allocate population fields with grid.create_field(..., padded=True) or create_nse_fields)")
+    return plane
+
+
+def shared_layout(fields):
+    """(plane, fields): the fields as given when they share one plane stride, else contiguous
+    copies of them (plane N).  Callers copy outputs back when the tensor object changed."""
+    planes = {plane_stride(t) for t in fields if t is not None}
+    if len(planes) == 1 and None not in planes:
+        return planes.pop(), list(fields)
+    out = [t.contiguous() if t is not None else None for t in fields]
+    return 0, out
+
+
+def common_plane(planes, what):
+    """The one plane stride shared by the population fields of a call."""
+    ps = {p for p in planes if p is not None}
+    if len(ps) > 1:
+        raise ValueError(f"{what}: population fields of one call must share one plane stride; got {sorted(ps)}")
+    return ps.pop() if ps else 0
 
 
 def expect_field(t, cardinality, extents, name, dtype=None):

@@ -21,6 +21,7 @@ from xlb_amd import _lib
 from xlb_amd.compute_backend import ComputeBackend
 from xlb_amd.default_config import DefaultConfig
 from xlb_amd.helper.check_boundary_overlaps import check_bc_overlaps
+from xlb_amd.grid.grid import copy_population
 from xlb_amd.helper.nse_solver import create_nse_fields
 from xlb_amd.operator.boundary_condition.boundary_condition import ImplementationStep
 from xlb_amd.operator.boundary_masker import IndicesBoundaryMasker, MeshBoundaryMasker
@@ -28,7 +29,7 @@ from xlb_amd.operator.boundary_masker.packing import cells_with_id, packed_missi
 from xlb_amd.operator.collision import BGK, KBC, ForcedCollision, SmagorinskyLESBGK
 from xlb_amd.operator.equilibrium import QuadraticEquilibrium
 from xlb_amd.operator.macroscopic import Macroscopic
-from xlb_amd.operator.operator import Operator, expect_field, field_extents
+from xlb_amd.operator.operator import Operator, common_plane, expect_field, expect_population, field_extents
 from xlb_amd.operator.stream import Stream
 
 
@@ -95,7 +96,10 @@ class IncompressibleNavierStokesStepper(Stepper):
             from xlb_amd.helper.initializers import initialize_eq
 
             f_0 = initialize_eq(f_0, self.grid, self.velocity_set, self.precision_policy, self.compute_backend)
-        f_1.copy_(f_0)
+        if f_1.shape == f_0.shape and f_1.stride() == f_0.stride():
+            copy_population(f_1, f_0)
+        else:
+            f_1.copy_(f_0)
         bc_mask, missing_mask = self._process_boundary_conditions(self.boundary_conditions, bc_mask, missing_mask)
         f_0, f_1 = self._initialize_auxiliary_data(self.boundary_conditions, f_0, f_1, bc_mask, missing_mask)
         self._sync_state(bc_mask, missing_mask)
@@ -193,11 +197,13 @@ class IncompressibleNavierStokesStepper(Stepper):
     def _launch_args(self, f_0, f_1, bc_mask, missing_mask):
         nx, ny, nz = field_extents(f_0)
         q, ext = self.velocity_set.q, (nx, ny, nz)
-        expect_field(f_0, q, ext, "f_0", self.store_dtype)
+        fmt = self.precision_policy.store_format
+        planes = [expect_population(f_0, q, ext, "f_0", self.store_dtype, self.lattice_id, fmt)]
         if f_1 is not None:
-            expect_field(f_1, q, ext, "f_1", self.store_dtype)
+            planes.append(expect_population(f_1, q, ext, "f_1", self.store_dtype, self.lattice_id, fmt))
             if f_0.data_ptr() == f_1.data_ptr():
                 raise ValueError("f_0 and f_1 must be distinct buffers")
+        plane = common_plane(planes, "IncompressibleNavierStokesStepper")
         expect_field(bc_mask, 1, ext, "bc_mask", torch.uint8)
         self._sync_state(bc_mask, missing_mask)
         force_mode, force_vec, force_field = self._force_args()
@@ -208,17 +214,17 @@ class IncompressibleNavierStokesStepper(Stepper):
             masks = (_lib.ptr(bc_mask), _lib.ptr(self._bits), _lib.ptr(self._aux), _lib.ptr(self._table))
         else:
             masks = (None, None, None, None)
-        return nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field
+        return nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field
 
     # ------------------------------------------------------------------ stepping
     @Operator.register_backend(ComputeBackend.MI355X)
     def mi355x_implementation(self, f_0, f_1, bc_mask, missing_mask, omega, timestep):
-        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
+        nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
         _lib.check(
             _lib.lib().xlbh_step(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
-                nx, ny, nz, float(omega), self._collision_param(), int(timestep), _lib.stream_of(f_0),
+                nx, ny, nz, plane, float(omega), self._collision_param(), int(timestep), _lib.stream_of(f_0),
             ),
             "IncompressibleNavierStokesStepper",
         )
@@ -255,7 +261,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         lam_out = dL/df_{t+1}, write lam_in = dL/df_t and add dL/dg to ``grad_force`` ((d, nx, ny, nz)
         float64) when the stepper has a body force.  Every collision (BGK, KBC, SmagorinskyLESBGK)."""
         self._check_adjoint_supported()
-        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_t, None, bc_mask, missing_mask)
+        nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_t, None, bc_mask, missing_mask)
         q, ext = self.velocity_set.q, (nx, ny, nz)
         expect_field(lam_out, q, ext, "lam_out", self.compute_dtype)
         expect_field(lam_in, q, ext, "lam_in", self.compute_dtype)
@@ -270,7 +276,7 @@ class IncompressibleNavierStokesStepper(Stepper):
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_t), _lib.ptr(lam_out), _lib.ptr(lam_in), *masks, *self._adjoint_cells(), force_mode, fu_ptr,
                 _lib.ptr(force_field),
-                _lib.ptr(grad_force), nx, ny, nz, float(omega), self._collision_param(), _lib.stream_of(f_t),
+                _lib.ptr(grad_force), nx, ny, nz, plane, float(omega), self._collision_param(), _lib.stream_of(f_t),
             ),
             "IncompressibleNavierStokesStepper.adjoint_step",
         )
@@ -281,23 +287,55 @@ class IncompressibleNavierStokesStepper(Stepper):
         """``num_states`` steps from ``f_start`` keeping every state (xlbh_record): states[j] is the
         output of step ``timestep + j``.  ``states`` is a contiguous (>= num_states, q, ...) tensor
         in the store dtype."""
-        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_start, None, bc_mask, missing_mask)
+        nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_start, None, bc_mask, missing_mask)
         if num_states < 0:
             raise ValueError("num_states must be >= 0")
-        if num_states > 0 and (states is None or states.dtype != self.store_dtype or not states.is_contiguous()
-                               or states.shape[0] < num_states or tuple(states.shape[1:]) != tuple(f_start.shape)):
-            raise ValueError("states must be a contiguous (num_states, q, nx, ny, nz) store-dtype tensor")
+        if num_states > 0:
+            self._expect_states(states, num_states, f_start, plane)
         _lib.check(
             _lib.lib().xlbh_record(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_start), _lib.ptr(states), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
-                nx, ny, nz, float(omega), self._collision_param(), int(timestep), int(num_states),
+                nx, ny, nz, plane, float(omega), self._collision_param(), int(timestep), int(num_states),
                 _lib.stream_of(f_start),
             ),
             "IncompressibleNavierStokesStepper.record",
         )
         _lib.mark_written(states)
         return states
+
+    def state_buffer(self, num_states, f_like):
+        """Storage for ``num_states`` population fields laid out like ``f_like`` (its plane stride),
+        one after another, as ``record`` and ``adjoint_run`` expect: a (num_states, q, nx, ny, nz)
+        view with strides (q * plane, plane, ny * nz, nz, 1)."""
+        from xlb_amd.operator.operator import plane_stride
+
+        q, nx, ny, nz = (int(v) for v in f_like.shape)
+        plane = plane_stride(f_like)
+        if plane is None:
+            raise ValueError("f_like must be C-ordered within each plane")
+        per = int(_lib.lib().xlbh_population_bytes(self.lattice_id, self.precision_policy.store_format, nx, ny, nz,
+                                                   plane))
+        elem = f_like.element_size()
+        if per < 0 or per % elem or (per // elem) % q:
+            raise ValueError("state buffers are not available for this store format")
+        buf = torch.empty(max(1, num_states) * per // elem, dtype=f_like.dtype, device=f_like.device)
+        return buf.as_strided((max(1, num_states), q, nx, ny, nz), (per // elem, plane, ny * nz, nz, 1))
+
+    def _expect_states(self, states, n, f_like, plane):
+        q = self.velocity_set.q
+        nx, ny, nz = (int(v) for v in f_like.shape[1:])
+        per = int(_lib.lib().xlbh_population_bytes(self.lattice_id, self.precision_policy.store_format, nx, ny, nz,
+                                                   plane))
+        elem = f_like.element_size()
+        want = (per // elem, plane, ny * nz, nz, 1)
+        ok = (states is not None and states.dtype == self.store_dtype and states.dim() == 5 and states.shape[0] >= n
+              and tuple(states.shape[1:]) == (q, nx, ny, nz) and per % elem == 0
+              and all(states.shape[d] == 1 or states.stride(d) == want[d] for d in range(5))
+              and states.untyped_storage().nbytes() - states.storage_offset() * elem >= n * per)
+        if not ok:
+            raise ValueError(f"states must hold {n} population fields laid out like the state they follow: a "
+                             f"(n, q, nx, ny, nz) store-dtype tensor with strides {want} (see state_buffer)")
 
     def adjoint_workspace(self, bc_mask):
         """A workspace for the fused reverse step (see ``adjoint_run``)."""
@@ -318,7 +356,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         self._check_adjoint_supported()
         if self._has_solid:
             raise NotImplementedError("the adjoint step does not support solid (255) cells")
-        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_ckpt, None, bc_mask, missing_mask)
+        nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_ckpt, None, bc_mask, missing_mask)
         q, ext = self.velocity_set.q, (nx, ny, nz)
         expect_field(lam_a, q, ext, "lam_a", self.compute_dtype)
         expect_field(lam_b, q, ext, "lam_b", self.compute_dtype)
@@ -327,9 +365,7 @@ class IncompressibleNavierStokesStepper(Stepper):
         if num_steps < 1:
             raise ValueError("num_steps must be >= 1")
         if num_steps > 1:
-            if (states is None or states.dtype != self.store_dtype or not states.is_contiguous()
-                    or states.shape[0] < num_steps - 1 or tuple(states.shape[1:]) != tuple(f_ckpt.shape)):
-                raise ValueError("states must be a contiguous (num_steps - 1, q, nx, ny, nz) store-dtype tensor")
+            self._expect_states(states, num_steps - 1, f_ckpt, plane)
         if grad_force is not None:
             expect_field(grad_force, self.velocity_set.d, ext, "grad_force", torch.float64)
         flags = (_lib.ADJ_RECOMPUTE if recompute else 0)
@@ -342,7 +378,7 @@ class IncompressibleNavierStokesStepper(Stepper):
             _lib.lib().xlbh_adjoint_run(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_ckpt), _lib.ptr(states), _lib.ptr(lam_a), _lib.ptr(lam_b), *masks, *self._adjoint_cells(),
-                force_mode, fu_ptr, _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, float(omega),
+                force_mode, fu_ptr, _lib.ptr(force_field), _lib.ptr(grad_force), nx, ny, nz, plane, float(omega),
                 self._collision_param(), int(timestep), int(num_steps), _lib.ptr(workspace), flags,
                 _lib.stream_of(f_ckpt),
             ),
@@ -355,12 +391,12 @@ class IncompressibleNavierStokesStepper(Stepper):
         """``num_steps`` steps with the buffer swap in C; returns (f_0, f_1) with the newest
         populations in f_0, i.e. the state the reference's loop
         ``f_0, f_1 = stepper(...); f_0, f_1 = f_1, f_0`` leaves after ``num_steps`` passes."""
-        nx, ny, nz, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
+        nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
         _lib.check(
             _lib.lib().xlbh_run(
                 self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
                 _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
-                nx, ny, nz, float(omega), self._collision_param(), int(timestep), int(num_steps),
+                nx, ny, nz, plane, float(omega), self._collision_param(), int(timestep), int(num_steps),
                 _lib.stream_of(f_0),
             ),
             "IncompressibleNavierStokesStepper.run",

@@ -7,6 +7,11 @@ The reference's members keep their names and order.  Added:
   deviation ``f_l - w_l`` in FP16 / BF16 with FP32 arithmetic.  A stored field of these policies
   holds deviations, so raw tensor values differ from ``f`` by the lattice weights; every operator
   of this package reads and writes them in that form.
+* ``Precision.INT16`` and ``PrecisionPolicy.FP32I16_BLOCK`` -- 16-bit block floating point with
+  FP32 arithmetic: int16 mantissas of ``f_l / w_l - 1`` and one exponent per row segment of 64 to
+  256 cells, kept after the last plane of the field's storage (include/xlb_hip.h,
+  XLBH_I16_BLOCK).  Its fields come from ``create_field(..., padded=True)`` /
+  ``create_nse_fields``; ``decode_populations`` turns one into FP32 values.
 """
 
 from enum import Enum, auto
@@ -21,6 +26,7 @@ class Precision(Enum):
     UINT8 = auto()
     BOOL = auto()
     BF16 = auto()
+    INT16 = auto()
 
     @property
     def torch_dtype(self):
@@ -31,6 +37,7 @@ class Precision(Enum):
             Precision.UINT8: torch.uint8,
             Precision.BOOL: torch.bool,
             Precision.BF16: torch.bfloat16,
+            Precision.INT16: torch.int16,
         }[self]
 
     @property
@@ -58,6 +65,7 @@ class PrecisionPolicy(Enum):
     FP32BF16 = auto()
     FP32FP16_SHIFTED = auto()
     FP32BF16_SHIFTED = auto()
+    FP32I16_BLOCK = auto()
 
     @property
     def compute_precision(self):
@@ -76,6 +84,7 @@ class PrecisionPolicy(Enum):
             PrecisionPolicy.FP32BF16: Precision.BF16,
             PrecisionPolicy.FP32FP16_SHIFTED: Precision.FP16,
             PrecisionPolicy.FP32BF16_SHIFTED: Precision.BF16,
+            PrecisionPolicy.FP32I16_BLOCK: Precision.INT16,
         }[self]
 
     @property
@@ -84,8 +93,15 @@ class PrecisionPolicy(Enum):
         return self in (PrecisionPolicy.FP32FP16_SHIFTED, PrecisionPolicy.FP32BF16_SHIFTED)
 
     @property
+    def block(self):
+        """True for the 16-bit block floating-point store (int16 mantissas, row-segment exponents)."""
+        return self == PrecisionPolicy.FP32I16_BLOCK
+
+    @property
     def store_format(self):
-        """Population storage format id in the C ABI (XLBH_F32 ... XLBH_BF16_SHIFTED)."""
+        """Population storage format id in the C ABI (XLBH_F32 ... XLBH_I16_BLOCK)."""
+        if self.block:
+            return 6
         base = {Precision.FP32: 0, Precision.FP64: 1, Precision.FP16: 2, Precision.BF16: 3}[self.store_precision]
         if self.shifted:
             base += 2

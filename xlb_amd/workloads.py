@@ -59,7 +59,7 @@ def periodic_box_3d(n, policy, omega=1.0, seed=0):
     rho = torch.ones((1, n, n, n), dtype=torch.float32, device=f0.device)
     _eq_state(st, f0, rho, u)
     del u, rho
-    f1.copy_(f0)
+    xa.grid.copy_population(f1, f0)
     return Workload(st, f0, f1, bm, mm, omega)
 
 
@@ -128,7 +128,7 @@ def taylor_green_3d(shape, policy, u0=0.04, nu=0.01):
     rho = torch.ones((1,) + tuple(shape), dtype=torch.float64, device=f0.device)
     _eq_state(st, f0, rho, u)
     del u, rho
-    f1.copy_(f0)
+    xa.grid.copy_population(f1, f0)
     return Workload(st, f0, f1, bm, mm, 1.0 / (3.0 * nu + 0.5))
 
 
