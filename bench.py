@@ -732,12 +732,22 @@ def main():
     log(f"bench: fp32 {fp32['mlups']:.0f} MLUPS ({fp32['achieved_gbs']:.0f} GB/s)")
     store16 = {}
     if not args.no_store16:
+        # In a kernel trace of this workload the first ~50 steps after the host's synchronisation
+        # run up to 17 % slower for the block-format kernel and up to 7 % for shifted FP16 (the FP32
+        # kernel's do not), then settle; a 20-step window would time that transient only.  The
+        # 16-bit rates, and the FP32 rate they are compared with, are therefore measured over a
+        # longer window of the same workload: 50 warm-up + 200 timed steps each.
+        w16, s16 = max(args.warmup, 50), max(args.steps, 200)
+        ref = fp32 if (s16, w16) == (args.steps, args.warmup) else headline_leg(
+            xa, torch, xa.PrecisionPolicy.FP32FP32, n, s16, w16, python_loop=False)
+        store16["window"] = dict(warmup=w16, steps=s16, fp32_mlups=ref["mlups"], fp32_kernel_ms=ref["kernel_ms"])
         for pol in (xa.PrecisionPolicy.FP32I16_BLOCK, xa.PrecisionPolicy.FP32FP16, xa.PrecisionPolicy.FP32FP16_SHIFTED,
                     xa.PrecisionPolicy.FP32BF16_SHIFTED):
-            r = headline_leg(xa, torch, pol, n, args.steps, args.warmup, python_loop=False)
-            log(f"bench: {pol.name} {r['mlups']:.0f} MLUPS ({r['achieved_gbs']:.0f} GB/s)")
+            r = headline_leg(xa, torch, pol, n, s16, w16, python_loop=False)
+            log(f"bench: {pol.name} {r['mlups']:.0f} MLUPS ({r['achieved_gbs']:.0f} GB/s), "
+                f"{r['mlups'] / ref['mlups']:.2f} x FP32 over the same window")
             store16[pol.name] = dict(mlups=r["mlups"], achieved_gbs=r["achieved_gbs"], kernel_ms=r["kernel_ms"],
-                                     speedup_vs_fp32=r["mlups"] / fp32["mlups"], frac=r["achieved_gbs"] / HBM_PEAK_GBS)
+                                     speedup_vs_fp32=r["mlups"] / ref["mlups"], frac=r["achieved_gbs"] / HBM_PEAK_GBS)
     copy = copy_leg(torch)
     log(f"bench: device copy {copy['gbs']:.0f} GB/s")
     pull = pull_pattern_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n)

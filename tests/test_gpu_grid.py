@@ -74,7 +74,7 @@ def test_padded_population_fields_index_like_contiguous_ones(policy):
     q, nx, ny, nz = f.shape
     plane = int(_lib.lib().xlbh_plane_stride(1, policy.store_format, nx, ny, nz))
     assert f.stride() == (plane, ny * nz, nz, 1) and plane >= nx * ny * nz
-    dense = f.contiguous()
+    dense = f.clone(memory_format=torch.contiguous_format)
     assert dense.stride() == (nx * ny * nz, ny * nz, nz, 1)
     np.testing.assert_array_equal(f.cpu().numpy(), dense.cpu().numpy())
     np.testing.assert_array_equal(f[3, 2:5, 1, ::2].cpu().numpy(), dense[3, 2:5, 1, ::2].cpu().numpy())
