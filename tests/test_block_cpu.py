@@ -102,3 +102,24 @@ def test_block_nonfinite_segment_decodes_to_inf():
     v = decode(a, orc.I16_BLOCK, w)  # float64: beyond the float range, i.e. +inf once rounded to float
     sel = (idx == bad)
     assert np.all(v[:, sel] > np.finfo(np.float32).max) and np.all(np.abs(v[:, ~sel]) < 1)
+
+
+@pytest.mark.parametrize("lat,shape3", [(1, (3, 4, 300)), (0, (6, 70, 1))])
+def test_package_decode_matches_the_format(lat, shape3):
+    """xlb_amd.grid.decode_populations (and block_exponents) read a block field the way the format
+    defines it: the same values as the test-side decoder, on a field the oracle wrote."""
+    import torch
+
+    from lbm_cases import _block_buffer
+
+    import xlb_amd as xa
+
+    q = LATS[lat]
+    w = orc.lattice(lat)["w"]
+    rho, u = _state(lat, shape3, seed=11)
+    a = pop_alloc(orc.I16_BLOCK, q, shape3)
+    orc.equilibrium(lat, orc.I16_BLOCK, 0, rho, u, a)
+    t = torch.from_numpy(_block_buffer(a))[: a.size].view(a.shape)
+    np.testing.assert_array_equal(xa.grid.block_exponents(t).numpy(), block_exps(a))
+    got = xa.grid.decode_populations(t, w, block=True).numpy()
+    np.testing.assert_allclose(got, decode(a, orc.I16_BLOCK, w), rtol=1e-15, atol=0)
