@@ -255,6 +255,19 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
                            const void* aux, double* out, void* workspace,
                            int nx, int ny, int nz, int64_t plane, void* stream);
 
+/* xlbh_momentum_transfer without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It calls no allocating routine: the per-block partials go to a
+ * 2 MiB array the library holds per device (for the process lifetime), so grids of at most 2^28
+ * cells are accepted (larger ones fail with a message).  Calls on one GPU share that array: calls
+ * that overlap on one GPU (different streams, or host threads) race and give wrong results; calls
+ * on one stream are ordered and safe.  Results are bit-identical to xlbh_momentum_transfer. */
+int xlbh_momentum_transfer_noalloc(int lattice, int store, int compute, int kind, int bc_id,
+                                   const xlbh_bc_desc* desc,
+                                   const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                                   const void* aux, double* out,
+                                   int nx, int ny, int nz, int64_t plane, void* stream);
+
 /* Global mass and momentum: out[0] = sum rho, out[1..3] = sum rho*u (= sum c f) over every
  * cell whose bc_mask is not 255 (bc_mask may be NULL). */
 int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask,

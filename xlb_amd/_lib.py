@@ -79,6 +79,7 @@ SIGNATURES = {
     "xlbh_mesh_mask": (_i, [_i, _vp, _i64, _i, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "xlbh_reduce_workspace_bytes": (_i64, [_i64]),
     "xlbh_momentum_transfer": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_momentum_transfer_noalloc": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_mass_momentum": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_adjoint_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
                                _i, _i64, _d, _d, _vp]),

@@ -4,6 +4,8 @@
 // then a wave64 shuffle tree and the 4 wave sums in LDS) into a per-block partial; pass 2 has one
 // block sum the partials in a fixed strided order + tree.  With 4096 cells per block a 256^3 grid
 // leaves 4096 partials, which one block sums in a few microseconds.
+#include <mutex>
+
 #include "physics.hpp"
 
 namespace xlbh {
@@ -199,12 +201,14 @@ int64_t xlbh_reduce_workspace_bytes(int64_t n_cells) {
   return ((n_cells + kRedCells - 1) / kRedCells) * kRedWidth * (int64_t)sizeof(double);
 }
 
-int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
-                           const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
-                           double* out, void* workspace, int nx, int ny, int nz, int64_t plane, void* stream) {
-  clear_error();
-  if (int e = check_grid(lattice, nx, ny, nz)) return e;
-  XLBH_REQUIRE(f && bc_mask && missing_bits && out && workspace, "momentum_transfer: null argument");
+}  // extern "C"
+
+// Momentum exchange with the per-block partials in `part` (red_blocks(N) x kRedWidth doubles).
+static int momentum_transfer_into(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
+                                  const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits,
+                                  const void* aux, double* out, double* part, int nx, int ny, int nz, int64_t plane,
+                                  void* stream) {
+  XLBH_REQUIRE(f && bc_mask && missing_bits && out && part, "momentum_transfer: null argument");
   uint64_t P;
   if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   const Sweep sw = make_sweep(nx, ny, nz);
@@ -215,7 +219,6 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
   if (desc) d = *desc;
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
   const int nb = (int)red_blocks(N);
-  double* part = static_cast<double*>(workspace);
   hipStream_t s = (hipStream_t)stream;
   return lat_dispatch(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
@@ -251,6 +254,61 @@ int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc
     set_error("unsupported (store, compute) combination");
     return 1;
   });
+}
+
+// Scratch of xlbh_momentum_transfer_noalloc: the per-block partials in a file-scope device array
+// (each GPU the library runs on holds its own copy, 2 MiB, until the process ends).  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in symbol, not a device
+// address): the entry point resolves each device's copy once with hipGetSymbolAddress, with that
+// device current, and passes that pointer.
+constexpr uint32_t kScratchParts = 1u << 16;  // partials: grids of up to 2^16 * 4096 = 2^28 cells
+static __device__ __align__(256) double scratch[kScratchParts * kRedWidth];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_partials(double** out) {
+  int dev = 0;
+  XLBH_HIP_CHECK(hipGetDevice(&dev));
+  XLBH_REQUIRE(dev >= 0 && dev < kMaxDevices, "momentum_transfer_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev] == nullptr) {
+    void* p = nullptr;
+    XLBH_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    XLBH_REQUIRE(p != nullptr, "momentum_transfer_noalloc: no device address for the scratch array");
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int xlbh_momentum_transfer(int lattice, int store, int compute, int kind, int bc_id, const xlbh_bc_desc* desc,
+                           const void* f, const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                           double* out, void* workspace, int nx, int ny, int nz, int64_t plane, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  return momentum_transfer_into(lattice, store, compute, kind, bc_id, desc, f, bc_mask, missing_bits, aux, out,
+                                static_cast<double*>(workspace), nx, ny, nz, plane, stream);
+}
+
+int xlbh_momentum_transfer_noalloc(int lattice, int store, int compute, int kind, int bc_id,
+                                   const xlbh_bc_desc* desc, const void* f, const uint8_t* bc_mask,
+                                   const uint32_t* missing_bits, const void* aux, double* out, int nx, int ny,
+                                   int nz, int64_t plane, void* stream) {
+  clear_error();
+  if (int e = check_grid(lattice, nx, ny, nz)) return e;
+  XLBH_REQUIRE(red_blocks((uint32_t)((int64_t)nx * ny * nz)) <= kScratchParts,
+               "momentum_transfer_noalloc: grid larger than its scratch allows (at most 2^28 cells); use "
+               "xlbh_momentum_transfer with a workspace");
+  double* part = nullptr;
+  if (int e = scratch_partials(&part)) return e;
+  return momentum_transfer_into(lattice, store, compute, kind, bc_id, desc, f, bc_mask, missing_bits, aux, out,
+                                part, nx, ny, nz, plane, stream);
 }
 
 int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_mask, double* out, void* workspace,
