@@ -374,6 +374,11 @@ __host__ __device__ inline float blk_escale(float inv_w, int k) { return u2f(f2u
 // rint_even(t) as int16 for |t| < 2^15: t + 1.5 * 2^23 lies in [2^23, 2^24), where the float
 // spacing is 1, so the addition rounds t to an integer held in the low mantissa bits.
 __host__ __device__ inline int16_t blk_round(float t) { return (int16_t)(f2u(t + 12582912.0f) & 0xffffu); }
+// The mantissa of x = fma(f, 1/w, -1) at exponent k, from x itself: x * 2^k is exact, so
+// rint_even(fma(x, 2^k, 1.5 * 2^23)) equals blk_round(fma(f, blk_escale(1/w, k), -2^k)) bit for bit.
+__device__ __forceinline__ int16_t blk_mantissa(float x, float pow2k) {
+  return (int16_t)(f2u(__builtin_fmaf(x, pow2k, 12582912.0f)) & 0xffffu);
+}
 
 // Largest of v over the wave's 64 lanes (all lanes active), as a uniform value.
 template <int CTRL, int ROWS>
@@ -420,6 +425,18 @@ __device__ __forceinline__ int blk_segment_k(float amax, int lbz) {
 }
 
 int check_grid(int lattice, int nx, int ny, int nz);
+
+// Host dispatch over the lattice id: fn(std::integral_constant<int, LAT>{}).
+template <class Fn>
+int with_lattice(int lat, Fn&& fn) {
+  switch (lat) {
+    case XLBH_D2Q9: return fn(std::integral_constant<int, XLBH_D2Q9>{});
+    case XLBH_D3Q19: return fn(std::integral_constant<int, XLBH_D3Q19>{});
+    case XLBH_D3Q27: return fn(std::integral_constant<int, XLBH_D3Q27>{});
+  }
+  set_error("unknown lattice id");
+  return 1;
+}
 // Plane stride (elements) of a population field: 0 selects the dense stride N; otherwise
 // plane >= N.  Returns 0 and sets `out`, or non-zero with the error set.
 int resolve_plane(int lattice, int store, int nx, int ny, int nz, int64_t plane, uint64_t& out);

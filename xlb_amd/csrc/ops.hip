@@ -19,15 +19,17 @@ __global__ __launch_bounds__(256) void k_stream(const R* __restrict__ fin, R* __
   int x, y, z;
   if (!sweep_cell(sw, x, y, z)) return;
   const int nx = sw.nx, ny = sw.ny, nz = sw.nz;
-  const O cell = ((O)x * ny + y) * nz + z;
+  uniform_slow_axes(T.d == 3, x, y);
+  const O row = T.d == 3 ? ((O)x * ny + y) * nz : (O)x * ny;
+  const O lane = T.d == 3 ? (O)z : (O)y;
   const size_t pb = (size_t)P * sizeof(R);
   const Planes<R, O> in{reinterpret_cast<const char*>(fin), pb}, out{reinterpret_cast<const char*>(fout), pb};
   const PullOffsets<O> po(x, y, z, nx, ny, nz);
   R v[T.q];
 #pragma unroll
-  for (int l = 0; l < T.q; ++l) v[l] = ld_stream(in.at(l, po.template src<LAT>(l)));
+  for (int l = 0; l < T.q; ++l) v[l] = ld_stream(in.at(l, po.template uni<LAT>(l), po.template lane<LAT>(l)));
 #pragma unroll
-  for (int l = 0; l < T.q; ++l) st_stream(out.at_w(l, cell), v[l]);
+  for (int l = 0; l < T.q; ++l) st_stream(out.at_w(l, row, lane), v[l]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -295,17 +297,6 @@ __global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc
 // ------------------------------------------------------------------------------------------
 // Host dispatch helpers.
 // ------------------------------------------------------------------------------------------
-template <class Fn>
-int with_lattice(int lat, Fn&& fn) {
-  switch (lat) {
-    case XLBH_D2Q9: return fn(std::integral_constant<int, XLBH_D2Q9>{});
-    case XLBH_D3Q19: return fn(std::integral_constant<int, XLBH_D3Q19>{});
-    case XLBH_D3Q27: return fn(std::integral_constant<int, XLBH_D3Q27>{});
-  }
-  set_error("unknown lattice id");
-  return 1;
-}
-
 // (store, compute) pairs accepted by the standalone operators: all stores with FP32 compute,
 // and F64/F32/F16 stores with FP64 compute.
 template <class Fn>

@@ -182,19 +182,6 @@ __global__ __launch_bounds__(256) void k_mass_momentum(const typename Fmt<FMT>::
 
 using namespace xlbh;
 
-namespace {
-template <class Fn>
-int lat_dispatch(int lat, Fn&& fn) {
-  switch (lat) {
-    case XLBH_D2Q9: return fn(std::integral_constant<int, XLBH_D2Q9>{});
-    case XLBH_D3Q19: return fn(std::integral_constant<int, XLBH_D3Q19>{});
-    case XLBH_D3Q27: return fn(std::integral_constant<int, XLBH_D3Q27>{});
-  }
-  set_error("unknown lattice id");
-  return 1;
-}
-}  // namespace
-
 extern "C" {
 
 int64_t xlbh_reduce_workspace_bytes(int64_t n_cells) {
@@ -220,7 +207,7 @@ static int momentum_transfer_into(int lattice, int store, int compute, int kind,
   const uint32_t N = (uint32_t)((int64_t)nx * ny * nz);
   const int nb = (int)red_blocks(N);
   hipStream_t s = (hipStream_t)stream;
-  return lat_dispatch(lattice, [&](auto L) -> int {
+  return with_lattice(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
     auto go = [&](auto S, auto c) -> int {
       constexpr int FMT = decltype(S)::value;
@@ -324,7 +311,7 @@ int xlbh_mass_momentum(int lattice, int store, const void* f, const uint8_t* bc_
   const int nb = (int)red_blocks(N);
   double* part = static_cast<double*>(workspace);
   hipStream_t s = (hipStream_t)stream;
-  return lat_dispatch(lattice, [&](auto L) -> int {
+  return with_lattice(lattice, [&](auto L) -> int {
     constexpr int LAT = decltype(L)::value;
     auto go = [&](auto S) -> int {
       constexpr int FMT = decltype(S)::value;

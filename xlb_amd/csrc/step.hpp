@@ -59,6 +59,30 @@ struct Planes {
     return reinterpret_cast<const R*>(base + (size_t)l * plane_bytes + cell * (O)sizeof(R));
   }
   __device__ __forceinline__ R* at_w(int l, O cell) const { return const_cast<R*>(at(l, cell)); }
+  // The same address from a wave-uniform cell offset `uni` and a per-lane one `lane` (their sum is
+  // the cell): plane base plus uniform part form one scalar pointer, and the lane part is the
+  // instruction's 32-bit vector offset, so the address costs no vector arithmetic.
+  __device__ __forceinline__ const R* at(int l, O uni, O lane) const {
+    if constexpr (sizeof(O) == 4) {
+      using gchar = const __attribute__((address_space(1))) char;
+      gchar* b = (gchar*)(base + (uint32_t)((uint32_t)l * (uint32_t)plane_bytes + uni * (O)sizeof(R)));
+      // an opaque scalar (global) pointer: keeps the compiler from re-associating the sum into a
+      // 64-bit vector add per access (the empty statement emits nothing)
+      asm("" : "+s"(b));
+      return reinterpret_cast<const R*>((const char*)(b + (uint32_t)(lane * (O)sizeof(R))));
+    }
+    return at(l, uni + lane);
+  }
+  // Stores: the lane offset is restated next to each store, so that the store's own block sees a
+  // 32-bit offset and selects the scalar-base form too.
+  __device__ __forceinline__ R* at_w(int l, O uni, O lane) const {
+    if constexpr (sizeof(O) == 4) {
+      uint32_t off = (uint32_t)(lane * (O)sizeof(R));
+      asm("" : "+v"(off));
+      return reinterpret_cast<R*>(const_cast<char*>(reinterpret_cast<const char*>(at(l, uni, (O)0))) + off);
+    }
+    return const_cast<R*>(at(l, uni, lane));
+  }
 };
 
 // Reads of stored populations other than the pulls: the own cell's pre-stream values and the
@@ -105,7 +129,29 @@ struct PullOffsets {
     constexpr LatTables T = make_tables(LAT);
     return xo[T.c[0][l] + 1] + yo[T.c[1][l] + 1] + zo[T.c[2][l] + 1];
   }
+  // src(l) split into its wave-uniform part (the slow axes) and its per-lane part (the row axis:
+  // z in 3-D, y in 2-D), for Planes::at(l, uni, lane).
+  template <int LAT>
+  __device__ __forceinline__ O uni(int l) const {
+    constexpr LatTables T = make_tables(LAT);
+    if constexpr (T.d == 3) return xo[T.c[0][l] + 1] + yo[T.c[1][l] + 1];
+    return xo[T.c[0][l] + 1];
+  }
+  template <int LAT>
+  __device__ __forceinline__ O lane(int l) const {
+    constexpr LatTables T = make_tables(LAT);
+    if constexpr (T.d == 3) return zo[T.c[2][l] + 1];
+    return yo[T.c[1][l] + 1] + zo[T.c[2][l] + 1];
+  }
 };
+
+// A wave covers 64 consecutive cells of one row (row segments are at least 64 cells), so the
+// slow coordinates are wave-uniform: x and y in 3-D, x in 2-D.  Stating it puts everything
+// derived from them in scalar registers.
+__device__ __forceinline__ void uniform_slow_axes(bool three, int& x, int& y) {
+  x = __builtin_amdgcn_readfirstlane(x);
+  if (three) y = __builtin_amdgcn_readfirstlane(y);
+}
 
 // Everything after the pull for one cell with boundary id `id` (!= 255), in place on the
 // post-stream populations f: streaming-step BC, collision (+ force), collision-step BC and the
@@ -171,7 +217,9 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) F[dd] = ff[(size_t)dd * p.N + cell];
     }
-    collide<LAT, COLL>(f, (C)p.omega, has_force, F, (C)p.cparam);
+    // 16-bit stores take the short FP32 form of the collision (their rounding is far coarser)
+    constexpr bool SHORT = sizeof(typename Fmt<FMT>::raw) == 2 && std::is_same<C, float>::value;
+    collide<LAT, COLL, C, SHORT>(f, (C)p.omega, has_force, F, (C)p.cparam);
   }
 
   if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
@@ -253,7 +301,11 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
     y = min(y, ny - 1);
     z = min(z, nz - 1);
   }
+  uniform_slow_axes(T.d == 3, x, y);
   const O cell = ((O)x * ny + y) * nz + z;
+  // cell = row + lane: the row's first cell (wave-uniform) and the offset along the row
+  const O row = T.d == 3 ? ((O)x * ny + y) * nz : (O)x * ny;
+  const O lane = T.d == 3 ? (O)z : (O)y;
   const size_t plane_bytes = (size_t)p.plane * sizeof(R);
   const Planes<R, O> fout{static_cast<const char*>(p.f_out), plane_bytes};
   PopIn<FMT, C, O> fin{{static_cast<const char*>(p.f_in), plane_bytes}, p.e_in, p.sw, 0};
@@ -280,35 +332,44 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
     fin.k_own = kk[0];  // the rest direction pulls from the cell itself
     int m[Q];
 #pragma unroll
-    for (int l = 0; l < Q; ++l) m[l] = ld_stream(fin.pl.at(l, po.template src<LAT>(l)));
+    for (int l = 0; l < Q; ++l) m[l] = ld_stream(fin.pl.at(l, po.template uni<LAT>(l), po.template lane<LAT>(l)));
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int l = 0; l < Q; ++l) f[l] = __builtin_fmaf((float)m[l], blk_dscale((float)T.w[l], kk[l]), (float)T.w[l]);
   } else {
 #pragma unroll
-    for (int l = 0; l < Q; ++l) f[l] = ld_pop<FMT, C>(fin.pl.at(l, po.template src<LAT>(l)), T.w[l]);
+    for (int l = 0; l < Q; ++l)
+      f[l] = ld_pop<FMT, C>(fin.pl.at(l, po.template uni<LAT>(l), po.template lane<LAT>(l)), T.w[l]);
   }
   asm volatile("" ::: "memory");
   if constexpr (!BLK) {
     if (id == 255) return;  // solid: no update (nse_stepper.py:356-358)
     cell_update<LAT, COLL, FMT, C, O>(f, id, cell, x, y, z, p, fin);
 #pragma unroll
-    for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout.at_w(l, cell), f[l], T.w[l]);
+    for (int l = 0; l < Q; ++l) st_pop<FMT, C>(fout.at_w(l, row, lane), f[l], T.w[l]);
   } else {
+    // Lanes that write nothing (outside the grid, or solid) run the fluid update on what they
+    // pulled, unbranched, and stay out of the exponent and the stores.
     const bool write = live && id != 255;
     if (write) cell_update<LAT, COLL, FMT, C, O>(f, id, cell, x, y, z, p, fin);
-    const int k = blk_segment_k(write ? blk_cell_max<LAT>(f) : 0.0f, p.sw.lbz);
+    // x_l = f_l / w_l - 1, once: its largest magnitude sets the segment's exponent and its
+    // scaled value is the mantissa.  A segment spans whole waves, so k is wave-uniform.
+    float amax = 0.0f;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      f[l] = __builtin_fmaf(f[l], (float)(1.0 / T.w[l]), -1.0f);
+      amax = __builtin_elementwise_maximum(amax, __builtin_fabsf(f[l]));
+    }
+    const int k = __builtin_amdgcn_readfirstlane(blk_segment_k(write ? amax : 0.0f, p.sw.lbz));
     if ((threadIdx.x & ((1u << p.sw.lbz) - 1)) == 0) p.e_out[blk_row_seg(p.sw, xr, yr, fb)] = k;
     if (write) {
       if (k != kBlkNonFinite) {
-        const float nk = -blk_pow2(k);
+        const float pk = blk_pow2(k);
 #pragma unroll
-        for (int l = 0; l < Q; ++l)
-          st_stream(fout.at_w(l, cell),
-                    blk_round(__builtin_fmaf(f[l], blk_escale((float)(1.0 / T.w[l]), k), nk)));
+        for (int l = 0; l < Q; ++l) st_stream(fout.at_w(l, row, lane), blk_mantissa(f[l], pk));
       } else {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) st_stream(fout.at_w(l, cell), (int16_t)32767);
+        for (int l = 0; l < Q; ++l) st_stream(fout.at_w(l, row, lane), (int16_t)32767);
       }
     }
   }
@@ -323,26 +384,16 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
 // lookups the extra waves hide, and -1 % on a periodic box.  D3Q27 would spill at 80 VGPRs.
 constexpr unsigned kStepLdsCap = 36 * 1024;
 
-template <int LAT, int COLL, int FMT, class C, bool WIDE>
-__device__ __forceinline__ void step_dispatch(const StepParams& p) {
-  if constexpr (FMT == XLBH_I16_BLOCK) {
-    if (blk_segs_fast(p.sw) > 1)
-      step_body<LAT, COLL, FMT, C, WIDE, true>(p);
-    else
-      step_body<LAT, COLL, FMT, C, WIDE, false>(p);
-  } else {
-    step_body<LAT, COLL, FMT, C, WIDE, false>(p);
-  }
-}
-
-template <int LAT, int COLL, int FMT, class C, bool WIDE>
+// MSEG (block format only): rows of more than one exponent segment.  Its own kernel, so each
+// form gets its own register allocation.
+template <int LAT, int COLL, int FMT, class C, bool WIDE, bool MSEG = false>
 __global__ __launch_bounds__(256) void k_step(StepParams p) {
-  step_dispatch<LAT, COLL, FMT, C, WIDE>(p);
+  step_body<LAT, COLL, FMT, C, WIDE, MSEG>(p);
 }
 
-template <int LAT, int COLL, int FMT, class C, bool WIDE>
+template <int LAT, int COLL, int FMT, class C, bool WIDE, bool MSEG = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_step_w6(StepParams p) {
-  step_dispatch<LAT, COLL, FMT, C, WIDE>(p);
+  step_body<LAT, COLL, FMT, C, WIDE, MSEG>(p);
 }
 
 
@@ -393,12 +444,23 @@ int launch_step_t(const StepCall& c) {
   const bool wide = (uint64_t)Lat<LAT>::Q * p.plane * sizeof(typename Fmt<FMT>::raw) > 0xffffffffull;
   const unsigned lds = sizeof(typename Fmt<FMT>::raw) >= 4 ? kStepLdsCap : 0u;
   constexpr bool w6 = LAT == XLBH_D3Q19 && sizeof(typename Fmt<FMT>::raw) == 2 && std::is_same<C, float>::value;
-  if (wide)
-    hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
-  else if constexpr (w6)
-    hipLaunchKernelGGL((k_step_w6<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
-  else
-    hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+  auto launch = [&](auto mseg) {
+    constexpr bool MS = decltype(mseg)::value;
+    if (wide)
+      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true, MS>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+    else if constexpr (w6)
+      hipLaunchKernelGGL((k_step_w6<LAT, COLL, FMT, C, false, MS>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+    else
+      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false, MS>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+  };
+  if constexpr (FMT == XLBH_I16_BLOCK) {
+    if (blk_segs_fast(p.sw) > 1)
+      launch(std::true_type{});
+    else
+      launch(std::false_type{});
+  } else {
+    launch(std::false_type{});
+  }
   XLBH_LAUNCH_CHECK();
   return 0;
 }
