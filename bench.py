@@ -139,8 +139,8 @@ def precision_leg(xa, torch, shape=(512, 256, 256), steps=2000):
     P = xa.PrecisionPolicy
     out, u64, ke0 = {}, None, None
     cells = int(np.prod(shape))
-    for pol in (P.FP64FP64, P.FP32FP32, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16,
-                P.FP32BF16):
+    for pol in (P.FP64FP64, P.FP32FP32, P.FP32FP32_DEVIATION, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED,
+                P.FP32BF16_SHIFTED, P.FP32FP16, P.FP32BF16):
         wl = taylor_green_3d(shape, pol)
         vs = wl.stepper.velocity_set
         if ke0 is None:
@@ -199,8 +199,8 @@ def headline_precision_leg(xa, torch, n=256, steps=1000):
 
     P = xa.PrecisionPolicy
     out, u64 = {}, None
-    for pol in (P.FP64FP64, P.FP32FP32, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED, P.FP32BF16_SHIFTED, P.FP32FP16,
-                P.FP32BF16):
+    for pol in (P.FP64FP64, P.FP32FP32, P.FP32FP32_DEVIATION, P.FP32I16_BLOCK, P.FP32FP16_SHIFTED,
+                P.FP32BF16_SHIFTED, P.FP32FP16, P.FP32BF16):
         wl = lid_driven_cavity_3d(n, pol)
         f0, _ = wl.stepper.run(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, steps)
         u = decoded_velocity(torch, f0, pol, wl.stepper.velocity_set)

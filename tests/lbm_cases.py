@@ -69,7 +69,10 @@ def pop_copy(a):
 
 
 def policy_codes(policy):
-    return policy.store_format, policy.compute_id
+    """(store format, compute id) of a policy for the oracle, which restates the reference's FP32
+    arithmetic also for the deviation form (it has no other)."""
+    comp = policy.compute_id
+    return policy.store_format, (0 if comp == 2 else comp)
 
 
 def to_np(t):

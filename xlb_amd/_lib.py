@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libxlbhip.so")
 D2Q9, D3Q19, D3Q27 = 0, 1, 2
 BGK, KBC, SMAGORINSKY_BGK = 0, 1, 2
 F32, F64, F16, BF16, F16_SHIFTED, BF16_SHIFTED, I16_BLOCK = 0, 1, 2, 3, 4, 5, 6
-COMPUTE_F32, COMPUTE_F64 = 0, 1
+COMPUTE_F32, COMPUTE_F64, COMPUTE_F32_DEVIATION = 0, 1, 2
 FIELD_F32, FIELD_F64, FIELD_F16, FIELD_BF16 = 0, 1, 2, 3
 BC_NONE = 0
 BC_EQUILIBRIUM = 1
@@ -80,6 +80,8 @@ SIGNATURES = {
     "xlbh_reduce_workspace_bytes": (_i64, [_i64]),
     "xlbh_momentum_transfer": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_momentum_transfer_noalloc": (_i, [_i, _i, _i, _i, _i, _bcp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_checkpoint_encode": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
+    "xlbh_checkpoint_decode": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_mass_momentum": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_adjoint_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _vp, _vp, _i, _i,
                                _i, _i64, _d, _d, _vp]),

@@ -3,9 +3,10 @@
 ``rollout(stepper, f0, bc_mask, missing_mask, omega, num_steps, force_field=None, ...)`` returns the
 populations after ``num_steps`` steps as a tensor that participates in torch autograd: gradients
 flow to ``f0`` and to the per-cell body-force field.  Memory is bounded by checkpointing: every
-``checkpoint_every`` steps (default ceil(sqrt(num_steps))) the state is kept, optionally in a
-16-bit shifted form (f - w stored as bfloat16); the backward pass recomputes each segment from
-its checkpoint and runs the adjoint step (xlbh_adjoint_step) through it in reverse.
+``checkpoint_every`` steps (default ceil(sqrt(num_steps))) the state is kept, optionally with its
+populations in bfloat16 (the library's checkpoint codec: the non-equilibrium part in bfloat16,
+rho and u in float32); the backward pass recomputes each segment from its checkpoint and runs
+the adjoint step (xlbh_adjoint_step) through it in reverse.
 
 The reference differentiates with Warp's tape over k-step segments with stored checkpoints
 (examples/out_of_core/autodiff_lbm.py:64-127, subroutine/autodiff_stepper_subroutine.py:287-555);
@@ -21,6 +22,7 @@ import math
 import numpy as np
 import torch
 
+from xlb_amd import _lib
 from xlb_amd.operator.collision import ForcedCollision
 
 
@@ -59,18 +61,38 @@ def macroscopic_torch(f, velocity_set):
 
 
 class _Checkpoint:
-    """State at a segment start: kept in the store dtype, or as bfloat16 of f - w_l."""
+    """State at a segment start.  ``mode``:
 
-    def __init__(self, f, mode, w):
+    * "store": a copy in the store dtype;
+    * "bf16": the checkpoint codec of the library (xlbh_checkpoint_encode): the non-equilibrium
+      part f_l - feq_l(rho, u) as bfloat16 plus rho - 1 and u as float32 ((2 Q + 4 (d + 1)) bytes per
+      cell; 0.71 of an FP32 field for D3Q19);
+    * "bf16_shifted": f - w_l as bfloat16 (half of an FP32 field; the rounding then also perturbs
+      rho and u, by about 2^-9 of |f - w_l|)."""
+
+    def __init__(self, f, mode, w, stepper):
         self.mode = mode
         if mode == "bf16":
+            q, ext = f.shape[0], tuple(f.shape[1:])
+            self.data = torch.empty((q,) + ext, dtype=torch.bfloat16, device=f.device)
+            self.moments = torch.empty((stepper.velocity_set.d + 1,) + ext, dtype=torch.float32, device=f.device)
+            _lib.check(_lib.lib().xlbh_checkpoint_encode(
+                stepper.lattice_id, stepper.precision_policy.store_format, _lib.ptr(f), _lib.ptr(self.data),
+                _lib.ptr(self.moments), *ext, f.stride(0), _lib.stream_of(f)), "checkpoint encode")
+        elif mode == "bf16_shifted":
             self.data = (f.float() - w).to(torch.bfloat16)
         else:
             self.data = f.clone()
 
-    def state(self, scratch, w):
-        """A store-dtype tensor holding the state (``scratch`` is filled for bf16 checkpoints)."""
+    def state(self, scratch, w, stepper):
+        """A store-dtype tensor holding the state (``scratch`` is filled for 16-bit checkpoints)."""
         if self.mode == "bf16":
+            _lib.check(_lib.lib().xlbh_checkpoint_decode(
+                stepper.lattice_id, stepper.precision_policy.store_format, _lib.ptr(self.data), _lib.ptr(self.moments),
+                _lib.ptr(scratch), *scratch.shape[1:], scratch.stride(0), _lib.stream_of(scratch)), "checkpoint decode")
+            _lib.mark_written(scratch)
+            return scratch
+        if self.mode == "bf16_shifted":
             scratch.copy_(self.data.float() + w)
             return scratch
         return self.data
@@ -94,7 +116,7 @@ class _Rollout(torch.autograd.Function):
         b = torch.empty_like(a)
         ckpts = {}
         for s in range(0, num_steps, every):
-            ckpts[s] = _Checkpoint(a, ckpt_mode, w)
+            ckpts[s] = _Checkpoint(a, ckpt_mode, w, stepper)
             a, b = stepper.run(a, b, bc_mask, missing_mask, omega, min(every, num_steps - s), timestep=s)
         ctx.stepper = stepper
         ctx.masks = (bc_mask, missing_mask)
@@ -124,7 +146,7 @@ class _Rollout(torch.autograd.Function):
         n, k = ctx.num_steps, ctx.every
         shape = tuple(lam_a.shape)
         starts = sorted(ctx.ckpts.keys(), reverse=True)
-        bf16 = any(c.mode == "bf16" for c in ctx.ckpts.values())
+        bf16 = any(c.mode != "store" for c in ctx.ckpts.values())
         nbuf = 2 if ctx.overlap else 1
         states = [torch.empty((max(1, min(k, n) - 1),) + shape, dtype=st.store_dtype, device=lam_a.device)
                   for _ in range(nbuf)]
@@ -141,7 +163,7 @@ class _Rollout(torch.autograd.Function):
             """Restore the checkpoint of segment idx and record its states into buffer idx % nbuf."""
             s = starts[idx]
             b = idx % nbuf
-            f_s = ctx.ckpts[s].state(scratch[b], ctx.w)
+            f_s = ctx.ckpts[s].state(scratch[b], ctx.w, st)
             st.record(f_s, states[b], bc_mask, missing_mask, ctx.omega, min(k, n - s) - 1, timestep=s)
             return f_s
 
@@ -187,8 +209,10 @@ def rollout(stepper, f0, bc_mask, missing_mask, omega, num_steps, force_field=No
             checkpoint_dtype="store", overlap="auto", fused="auto"):
     """Differentiable ``num_steps``-step rollout; see the module docstring.
 
-    checkpoint_dtype: "store" keeps checkpoints in the stepper's store dtype, "bf16" keeps f - w as
-    bfloat16 (half the memory; the recomputed trajectory then starts from the rounded state).
+    checkpoint_dtype: "store" keeps checkpoints in the stepper's store dtype; "bf16" keeps the
+    non-equilibrium part f - feq(rho, u) as bfloat16 with rho and u in float32 (0.71 of an FP32
+    field for D3Q19); "bf16_shifted" keeps f - w as bfloat16 (half the memory, coarser).  The
+    recomputed trajectory starts from the decoded state.
     overlap: record segment i-1 on a second stream while the reverse sweep of segment i runs
     (True / False / "auto" = grids of at most OVERLAP_MAX_CELLS cells); it doubles the recorded
     state workspace.
@@ -199,8 +223,8 @@ def rollout(stepper, f0, bc_mask, missing_mask, omega, num_steps, force_field=No
     if num_steps < 1:
         raise ValueError("num_steps must be >= 1")
     every = checkpoint_every or max(1, int(math.ceil(math.sqrt(num_steps))))
-    if checkpoint_dtype not in ("store", "bf16"):
-        raise ValueError("checkpoint_dtype must be 'store' or 'bf16'")
+    if checkpoint_dtype not in ("store", "bf16", "bf16_shifted"):
+        raise ValueError("checkpoint_dtype must be 'store', 'bf16' or 'bf16_shifted'")
     small = f0[0].numel() <= OVERLAP_MAX_CELLS
     if overlap == "auto":
         overlap = small

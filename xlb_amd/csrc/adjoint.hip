@@ -10,6 +10,7 @@ using namespace xlbh;
 static int adjoint_dispatch(int lattice, int store, int compute, const AdjParams& p, hipStream_t s,
                             int mode = ADJ_SPLIT, uint64_t ws_words = 0, uint8_t* flag = nullptr,
                             void* lam = nullptr) {
+  compute = arith_compute(compute);  // the reverse kernels of the deviation form are the FP32 ones
   switch (lattice) {
     case XLBH_D2Q9: return adjoint_dispatch_d2q9(store, compute, p, s, mode, ws_words, flag, lam);
     case XLBH_D3Q19: return adjoint_dispatch_d3q19(store, compute, p, s, mode, ws_words, flag, lam);

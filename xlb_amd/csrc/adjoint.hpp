@@ -61,9 +61,13 @@ __device__ __forceinline__ Dual<C>& operator*=(Dual<C>& a, Dual<C> b) { return a
 // Collision adjoint (BGK + optional force), in place: lam (Q, the post-collision adjoint) ->
 // G_h (Q).  Also returns dL/dg in gg (D values).
 // ------------------------------------------------------------------------------------------
+// The weights enter as w_l = k_l / D (small integers k_l, one division by D per moment): with
+// float weights A and B would carry a fixed relative rounding (sum_l float(w_l) - 1 is ~1.5e-8
+// for D3Q19), which the reverse sweep would apply coherently in every step.
 template <int LAT, class C>
 __device__ __forceinline__ void eq_moments(const C* lam, const C* u, const C* v, C& A, C* B) {
   constexpr LatTables T = make_tables(LAT);
+  constexpr int D = weight_den(LAT);
   A = C(0);
   B[0] = B[1] = B[2] = C(0);
   C vv = C(0);
@@ -74,11 +78,14 @@ __device__ __forceinline__ void eq_moments(const C* lam, const C* u, const C* v,
     C cv = C(0);
 #pragma unroll
     for (int d = 0; d < T.d; ++d) cv += C(T.c[d][l]) * v[d];
-    const C lw = lam[l] * C(T.w[l]);
+    const C lw = lam[l] * C((int)(T.w[l] * D + 0.5));
     A += lw * (C(1) + C(3) * cv + C(4.5) * cv * cv - C(1.5) * vv);
 #pragma unroll
     for (int d = 0; d < T.d; ++d) B[d] += lw * (C(3) * C(T.c[d][l]) + C(9) * cv * C(T.c[d][l]) - C(3) * v[d]);
   }
+  A = A / C(D);
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) B[d] = B[d] / C(D);
   (void)u;
 }
 

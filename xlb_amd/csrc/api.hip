@@ -65,7 +65,8 @@ static int validate_step(int lattice, int collision, int store, int compute, con
   XLBH_REQUIRE(collision == XLBH_BGK || collision == XLBH_KBC || collision == XLBH_SMAGORINSKY_BGK,
                "unknown collision id");
   XLBH_REQUIRE(valid_store(store), "unknown store format");
-  XLBH_REQUIRE(compute == XLBH_COMPUTE_F32 || compute == XLBH_COMPUTE_F64, "unknown compute type");
+  XLBH_REQUIRE(compute == XLBH_COMPUTE_F32 || compute == XLBH_COMPUTE_F64 || compute == XLBH_COMPUTE_F32_DEVIATION,
+               "unknown compute type");
   XLBH_REQUIRE(f_a && f_b, "population fields must be non-null");
   XLBH_REQUIRE(f_a != f_b, "the step cannot run in place: f_out must not alias f_in");
   XLBH_REQUIRE(!bc_mask || (missing_bits && bc_table), "bc_mask given without missing bits or BC table");

@@ -426,6 +426,12 @@ __device__ __forceinline__ int blk_segment_k(float amax, int lbz) {
 
 int check_grid(int lattice, int nx, int ny, int nz);
 
+// The arithmetic type of a compute id: XLBH_COMPUTE_F32_DEVIATION computes in FP32 (only the fused
+// step's collision distinguishes it).
+inline int arith_compute(int compute) {
+  return compute == XLBH_COMPUTE_F32_DEVIATION ? XLBH_COMPUTE_F32 : compute;
+}
+
 // Host dispatch over the lattice id: fn(std::integral_constant<int, LAT>{}).
 template <class Fn>
 int with_lattice(int lat, Fn&& fn) {

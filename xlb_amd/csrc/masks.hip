@@ -201,6 +201,7 @@ int xlbh_scatter_aux(int compute, const int32_t* cells, const void* values, int 
   XLBH_REQUIRE(cells && values && aux, "scatter_aux needs cells, values and aux");
   hipStream_t s = (hipStream_t)stream;
   const dim3 g((n + 255) / 256);
+  compute = arith_compute(compute);
   if (compute == XLBH_COMPUTE_F64)
     hipLaunchKernelGGL(k_scatter<double>, g, dim3(256), 0, s, cells, (const double*)values, n, (double*)aux);
   else if (compute == XLBH_COMPUTE_F32)

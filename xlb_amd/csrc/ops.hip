@@ -301,6 +301,7 @@ __global__ __launch_bounds__(256) void k_apply_bc(int kind, int id, xlbh_bc_desc
 // and F64/F32/F16 stores with FP64 compute.
 template <class Fn>
 int with_precision(int store, int compute, Fn&& fn) {
+  compute = arith_compute(compute);
   if (compute == XLBH_COMPUTE_F32) {
     switch (store) {
       case XLBH_F32: return fn(std::integral_constant<int, XLBH_F32>{}, float{});
@@ -325,7 +326,7 @@ int with_precision(int store, int compute, Fn&& fn) {
 template <class Fn>
 int with_precision_blk(int store, int compute, Fn&& fn) {
   if (store == XLBH_I16_BLOCK) {
-    if (compute == XLBH_COMPUTE_F32) return fn(std::integral_constant<int, XLBH_I16_BLOCK>{}, float{});
+    if (arith_compute(compute) == XLBH_COMPUTE_F32) return fn(std::integral_constant<int, XLBH_I16_BLOCK>{}, float{});
     set_error("the block format (XLBH_I16_BLOCK) computes in FP32 only");
     return 1;
   }

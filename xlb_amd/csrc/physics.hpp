@@ -144,6 +144,87 @@ __device__ __forceinline__ void bgk_paired(C* f, C rho, const C* u, C usqr, C om
   }
 }
 
+// Internal collision id, not an ABI value: BGK evaluated on deviations from the rest state
+// (XLBH_COMPUTE_F32_DEVIATION).
+constexpr int kBgkDeviation = 100;
+
+// Every lattice weight is k_l / D with a small integer k_l.
+constexpr int weight_den(int lat) { return lat == XLBH_D3Q27 ? 216 : 36; }
+
+// BGK in FP32 on deviations from the rest state (bgk.py:30-34 and exact_difference_force.py:71-86,
+// rearranged so that every rounding is that of a small quantity):
+//   g_l = f_l - w_l (exact for w_l / 2 <= f_l <= 2 w_l), drho = sum_l g_l, j = sum_l c_l g_l
+//   (accumulated in FP64, where these sums of FP32 values are exact),
+//   rho = 1 + drho, u = j / rho, cu_l = 3 c_l.u,
+//   geq_l = feq_l - w_l = (k_l / D) (drho + rho (cu_l (1 + cu_l / 2) - 1.5 |u|^2)),
+//   f_l <- w_l + (g_l + omega (geq_l - g_l)).
+// The weight k_l / D is applied as a float pair (hi + lo, exact to about 2^-48): with float weights
+// the equilibrium's mass and momentum are off by a fixed relative ~1e-8 in every step, an error
+// that adds up coherently over a long run.  omega multiplies only geq - g, which carries no mass
+// or momentum, so its rounding shifts the viscosity by ~1e-8 and conserves both.  The force term
+// feq(rho, u + F) - feq(rho, u) is written as rho w_l (cF + cF (cu + cF / 2) - 1.5 (2 u.F + |F|^2)),
+// cF = 3 c_l.F.
+template <int LAT>
+__device__ __forceinline__ void bgk_deviation(float* f, double omega, bool has_force, const float* F) {
+  constexpr LatTables T = make_tables(LAT);
+  constexpr int Q = T.q;
+  constexpr int D = weight_den(LAT);
+  // exact moment sums: u and rho - 1 carry one rounding each
+  float g[Q];
+  double dr = 0.0, jd[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    g[l] = f[l] - (float)T.w[l];
+    dr += (double)g[l];
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) {
+      if (T.c[d][l] == 1) jd[d] += (double)g[l];
+      else if (T.c[d][l] == -1) jd[d] -= (double)g[l];
+    }
+  }
+  const float drho = (float)dr;
+  const float rho = (float)(1.0 + dr);
+  float u[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int d = 0; d < T.d; ++d) u[d] = (float)(jd[d] / (1.0 + dr));
+  const float us = usqr15<LAT>(u);
+  const float om = (float)omega;
+  float fu = 0.0f, ff = 0.0f;
+  if (has_force) {
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) {
+      fu += u[d] * F[d];
+      ff += F[d] * F[d];
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < Q; ++l) {
+    const double wk = (double)(int)(T.w[l] * D + 0.5) / (double)D;  // w_l, exactly k_l / D
+    const float hi = (float)wk, lo = (float)(wk - (double)hi);
+    float cu = 0.0f, cf = 0.0f;
+#pragma unroll
+    for (int d = 0; d < T.d; ++d) {
+      if (T.c[d][l] == 1) {
+        cu += u[d];
+        cf += F[d];
+      } else if (T.c[d][l] == -1) {
+        cu -= u[d];
+        cf -= F[d];
+      }
+    }
+    cu *= 3.0f;
+    const float y = __builtin_fmaf(rho, __builtin_fmaf(0.5f * cu, cu, cu) - us, drho);
+    const float geq = __builtin_fmaf(y, hi, y * lo);
+    float gn = __builtin_fmaf(om, geq - g[l], g[l]);
+    if (has_force) {
+      cf *= 3.0f;
+      const float df = cf + cf * (cu + 0.5f * cf) - 1.5f * (2.0f * fu + ff);
+      gn = __builtin_fmaf(rho * (float)T.w[l], df, gn);
+    }
+    f[l] = (float)T.w[l] + gn;
+  }
+}
+
 // In-place collision f <- collide(f) with optional exact-difference body force F (delta u).
 //   BGK: f - omega (f - feq)                                      bgk.py:30-34
 //   KBC: f - beta (2 ds + gamma dh)                               kbc.py:268-296

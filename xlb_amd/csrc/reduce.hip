@@ -222,6 +222,7 @@ static int momentum_transfer_into(int lattice, int store, int compute, int kind,
       XLBH_LAUNCH_CHECK();
       return 0;
     };
+    compute = arith_compute(compute);
     if (compute == XLBH_COMPUTE_F32) {
       switch (store) {
         case XLBH_F32: return go(std::integral_constant<int, XLBH_F32>{}, float{});

@@ -217,9 +217,13 @@ __device__ __forceinline__ void cell_update(C* f, int id, O cell, int x, int y, 
 #pragma unroll
       for (int dd = 0; dd < T.d; ++dd) F[dd] = ff[(size_t)dd * p.N + cell];
     }
-    // 16-bit stores take the short FP32 form of the collision (their rounding is far coarser)
-    constexpr bool SHORT = sizeof(typename Fmt<FMT>::raw) == 2 && std::is_same<C, float>::value;
-    collide<LAT, COLL, C, SHORT>(f, (C)p.omega, has_force, F, (C)p.cparam);
+    if constexpr (COLL == kBgkDeviation) {
+      bgk_deviation<LAT>(f, p.omega, has_force, F);
+    } else {
+      // 16-bit stores take the short FP32 form of the collision (their rounding is far coarser)
+      constexpr bool SHORT = sizeof(typename Fmt<FMT>::raw) == 2 && std::is_same<C, float>::value;
+      collide<LAT, COLL, C, SHORT>(f, (C)p.omega, has_force, F, (C)p.cparam);
+    }
   }
 
   if (kind == XLBH_BC_EXTRAPOLATION_OUTFLOW) {
@@ -492,6 +496,11 @@ int launch_step_lat(const StepCall& c) {
       case XLBH_F16: return launch_step_t<LAT, COLL, XLBH_F16, double>(c);                      \
       default: break;                                                                           \
     }                                                                                           \
+  }
+  if (c.compute == XLBH_COMPUTE_F32_DEVIATION) {
+    if (c.collision == XLBH_BGK && c.store == XLBH_F32) return launch_step_t<LAT, kBgkDeviation, XLBH_F32, float>(c);
+    set_error("XLBH_COMPUTE_F32_DEVIATION is implemented for the BGK collision with FP32 storage");
+    return 1;
   }
   if (c.collision == XLBH_BGK) {
     XLBH_STEP_CASES(XLBH_BGK)

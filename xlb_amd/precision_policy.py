@@ -12,6 +12,12 @@ The reference's members keep their names and order.  Added:
   256 cells, kept after the last plane of the field's storage (include/xlb_hip.h,
   XLBH_I16_BLOCK).  Its fields come from ``create_field(..., padded=True)`` /
   ``create_nse_fields``; ``decode_populations`` turns one into FP32 values.
+* ``PrecisionPolicy.FP32FP32_DEVIATION`` -- FP32 storage and arithmetic, with the fused step's BGK
+  collision evaluated on deviations from the rest state (f_l - w_l, rho - 1), the cell's moment
+  sums accumulated exactly and the lattice weights applied without a systematic rounding
+  (include/xlb_hip.h, XLBH_COMPUTE_F32_DEVIATION): the same update as FP32FP32 with roundings
+  that do not add up coherently over long runs.  BGK only; every other operator computes as
+  FP32FP32.
 """
 
 from enum import Enum, auto
@@ -66,6 +72,7 @@ class PrecisionPolicy(Enum):
     FP32FP16_SHIFTED = auto()
     FP32BF16_SHIFTED = auto()
     FP32I16_BLOCK = auto()
+    FP32FP32_DEVIATION = auto()
 
     @property
     def compute_precision(self):
@@ -85,6 +92,7 @@ class PrecisionPolicy(Enum):
             PrecisionPolicy.FP32FP16_SHIFTED: Precision.FP16,
             PrecisionPolicy.FP32BF16_SHIFTED: Precision.BF16,
             PrecisionPolicy.FP32I16_BLOCK: Precision.INT16,
+            PrecisionPolicy.FP32FP32_DEVIATION: Precision.FP32,
         }[self]
 
     @property
@@ -109,5 +117,8 @@ class PrecisionPolicy(Enum):
 
     @property
     def compute_id(self):
-        """Compute type id in the C ABI (XLBH_COMPUTE_F32 / XLBH_COMPUTE_F64)."""
+        """Compute type id in the C ABI (XLBH_COMPUTE_F32 / XLBH_COMPUTE_F64 /
+        XLBH_COMPUTE_F32_DEVIATION)."""
+        if self == PrecisionPolicy.FP32FP32_DEVIATION:
+            return 2
         return 1 if self.compute_precision == Precision.FP64 else 0
