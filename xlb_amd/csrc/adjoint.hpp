@@ -438,7 +438,7 @@ __device__ __forceinline__ void adj_pull(const AdjParams& p, int x, int y, int z
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
 #pragma unroll
   for (int l = 0; l < Q; ++l) {
-    const int sx = wrap(x - T.c[0][l], nx), sy = wrap(y - T.c[1][l], ny), sz = wrap(z - T.c[2][l], nz);
+    const int sx = wrap1(x - T.c[0][l], nx), sy = wrap1(y - T.c[1][l], ny), sz = wrap1(z - T.c[2][l], nz);
     fps[l] = ld_pop<FMT, C>(fin + (size_t)l * p.P + ((uint32_t)sx * ny + sy) * nz + sz, T.w[l]);
   }
 }
@@ -534,6 +534,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
   constexpr int Q = T.q;
   int x, y, z;
   if (!sweep_cell(p.sw, x, y, z)) return;
+  uniform_slow_axes(T.d == 3, x, y);
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
   const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
   int id = p.bc_mask ? p.bc_mask[cell] : 0;
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(256) void k_adjoint_push(AdjParams p) {
   C* li = static_cast<C*>(p.lam_in);
 #pragma unroll
   for (int k = 0; k < Q; ++k) {
-    const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+    const int sx = wrap1(x - T.c[0][k], nx), sy = wrap1(y - T.c[1][k], ny), sz = wrap1(z - T.c[2][k], nz);
     li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = G[k];
   }
 }
@@ -659,7 +660,7 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
       grads_cell_adjoint<LAT>(fps, pre, mm, G, Gf, Gpre);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+        const int sx = wrap1(x - T.c[0][k], nx), sy = wrap1(y - T.c[1][k], ny), sz = wrap1(z - T.c[2][k], nz);
         add(k, ((uint32_t)sx * ny + sy) * nz + sz, Gf[k]);
         add(k, cell, Gpre[k]);
       }
@@ -687,7 +688,7 @@ __device__ __forceinline__ void adjoint_bc_cell(const AdjParams& p, uint32_t cel
       C s = C(0);
 #pragma unroll
       for (int l = 0; l < Q; ++l) s += G[l] * hd[l].d;
-      const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+      const int sx = wrap1(x - T.c[0][k], nx), sy = wrap1(y - T.c[1][k], ny), sz = wrap1(z - T.c[2][k], nz);
       add(k, ((uint32_t)sx * ny + sy) * nz + sz, s);
     }
   }
@@ -740,6 +741,7 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
   constexpr int Q = T.q;
   int x, y, z;
   if (!sweep_cell(p.sw, x, y, z)) return;
+  uniform_slow_axes(T.d == 3, x, y);
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
   const uint32_t cell = ((uint32_t)x * ny + y) * nz + z;
   // Every load that needs only the cell's position goes out first, in one batch: the boundary id,
@@ -761,7 +763,7 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
     // consistent regardless: deltas addressed to the cell are consumed like anywhere else.)
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+      const int sx = wrap1(x - T.c[0][k], nx), sy = wrap1(y - T.c[1][k], ny), sz = wrap1(z - T.c[2][k], nz);
       li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = C(0);
     }
     if (nbf) {
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(256) void k_adjoint_fused(AdjParams p) {
   }
 #pragma unroll
   for (int k = 0; k < Q; ++k) {
-    const int sx = wrap(x - T.c[0][k], nx), sy = wrap(y - T.c[1][k], ny), sz = wrap(z - T.c[2][k], nz);
+    const int sx = wrap1(x - T.c[0][k], nx), sy = wrap1(y - T.c[1][k], ny), sz = wrap1(z - T.c[2][k], nz);
     li[(size_t)k * p.N + ((uint32_t)sx * ny + sy) * nz + sz] = G[k];
   }
   if (own_kind(kind)) {
