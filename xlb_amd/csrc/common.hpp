@@ -277,22 +277,25 @@ inline dim3 sweep_grid(const Sweep& s) {
   return dim3((s.ny + bz - 1) / bz, (s.nx + by - 1) / by, 1);
 }
 
-// Thread -> cell coordinates; false when the thread lies outside the grid.
-__device__ __forceinline__ bool sweep_cell(const Sweep& s, int& x, int& y, int& z) {
+// Thread -> cell coordinates of tile (bx, by, bz); false when the thread lies outside the grid.
+__device__ __forceinline__ bool sweep_cell_of(const Sweep& s, int bx, int by, int bzi, int& x, int& y, int& z) {
   const int bz = 1 << s.lbz;
   const int t = threadIdx.x;
-  const int fast = blockIdx.x * bz + (t & (bz - 1));
-  const int mid = blockIdx.y * (256 >> s.lbz) + (t >> s.lbz);
+  const int fast = bx * bz + (t & (bz - 1));
+  const int mid = by * (256 >> s.lbz) + (t >> s.lbz);
   if (s.nz > 1) {
     z = fast;
     y = mid;
-    x = blockIdx.z;
+    x = bzi;
     return z < s.nz && y < s.ny;
   }
   z = 0;
   y = fast;
   x = mid;
   return y < s.ny && x < s.nx;
+}
+__device__ __forceinline__ bool sweep_cell(const Sweep& s, int& x, int& y, int& z) {
+  return sweep_cell_of(s, blockIdx.x, blockIdx.y, blockIdx.z, x, y, z);
 }
 
 __device__ __forceinline__ int wrap(int v, int n) {
@@ -422,6 +425,59 @@ __device__ __forceinline__ int blk_segment_k(float amax, int lbz) {
     if (lbz > 7) b = max(b, max(red[first + 2], red[first + 3]));
   }
   return blk_k_from_max(u2f(b));
+}
+
+// Population field addressing.  Plane l of a field starts at base + l * plane_bytes (uniform,
+// held in scalar registers); a cell is reached with an unsigned per-lane offset.  With
+// O = uint32_t (every plane below 4 GiB) the compiler emits the scalar-base + 32-bit vector-offset
+// form of global loads/stores; O = uint64_t serves larger planes.
+#ifndef XLBH_ONE_BASE
+#define XLBH_ONE_BASE 1
+#endif
+template <class R, class O>
+struct Planes {
+  const char* base;
+  size_t plane_bytes;
+  __device__ __forceinline__ const R* at(int l, O cell) const {
+#if XLBH_ONE_BASE
+    if constexpr (sizeof(O) == 4)
+      return reinterpret_cast<const R*>(base + (uint32_t)((uint32_t)l * (uint32_t)plane_bytes + cell * (O)sizeof(R)));
+#endif
+    return reinterpret_cast<const R*>(base + (size_t)l * plane_bytes + cell * (O)sizeof(R));
+  }
+  __device__ __forceinline__ R* at_w(int l, O cell) const { return const_cast<R*>(at(l, cell)); }
+  // The same address from a wave-uniform cell offset `uni` and a per-lane one `lane` (their sum is
+  // the cell): plane base plus uniform part form one scalar pointer, and the lane part is the
+  // instruction's 32-bit vector offset, so the address costs no vector arithmetic.
+  __device__ __forceinline__ const R* at(int l, O uni, O lane) const {
+    if constexpr (sizeof(O) == 4) {
+      using gchar = const __attribute__((address_space(1))) char;
+      gchar* b = (gchar*)(base + (uint32_t)((uint32_t)l * (uint32_t)plane_bytes + uni * (O)sizeof(R)));
+      // an opaque scalar (global) pointer: keeps the compiler from re-associating the sum into a
+      // 64-bit vector add per access (the empty statement emits nothing)
+      asm("" : "+s"(b));
+      return reinterpret_cast<const R*>((const char*)(b + (uint32_t)(lane * (O)sizeof(R))));
+    }
+    return at(l, uni + lane);
+  }
+  // Stores: the lane offset is restated next to each store, so that the store's own block sees a
+  // 32-bit offset and selects the scalar-base form too.
+  __device__ __forceinline__ R* at_w(int l, O uni, O lane) const {
+    if constexpr (sizeof(O) == 4) {
+      uint32_t off = (uint32_t)(lane * (O)sizeof(R));
+      asm("" : "+v"(off));
+      return reinterpret_cast<R*>(const_cast<char*>(reinterpret_cast<const char*>(at(l, uni, (O)0))) + off);
+    }
+    return const_cast<R*>(at(l, uni, lane));
+  }
+};
+
+// A wave covers 64 consecutive cells of one row (row segments are at least 64 cells), so the
+// slow coordinates are wave-uniform: x and y in 3-D, x in 2-D.  Stating it puts everything
+// derived from them in scalar registers.
+__device__ __forceinline__ void uniform_slow_axes(bool three, int& x, int& y) {
+  x = __builtin_amdgcn_readfirstlane(x);
+  if (three) y = __builtin_amdgcn_readfirstlane(y);
 }
 
 int check_grid(int lattice, int nx, int ny, int nz);

@@ -15,60 +15,69 @@ struct Grad {
   T xx, xy, xz, yx, yy, yz, zx, zy, zz;  // d u_a / d b as u_a_db
 };
 
-// Returns false when the cell is on the domain edge or next to a boundary cell.
-template <class T>
+// Returns false when the cell is on the domain edge or next to a boundary cell.  x and y are
+// wave-uniform (uniform_slow_axes), so every neighbour's address is a scalar row base plus the
+// lane's z offset (Planes::at(l, uni, lane)); O = uint32_t while the three velocity planes stay
+// below 4 GiB.
+template <class T, class O>
 __device__ __forceinline__ bool central_gradient(const T* u, const uint8_t* bc_mask, int x, int y, int z, int nx,
                                                  int ny, int nz, uint64_t N, Grad<T>& g) {
   if (x < 1 || y < 1 || z < 1 || x > nx - 2 || y > ny - 2 || z > nz - 2) return false;
-  auto at = [&](int a, int b, int c) { return ((uint64_t)a * ny + b) * nz + c; };
+  const O row = ((O)x * ny + y) * nz, dx = (O)ny * nz, dy = (O)nz;
+  const O zc = (O)z;
   if (bc_mask) {
-    if (bc_mask[at(x + 1, y, z)] | bc_mask[at(x, y + 1, z)] | bc_mask[at(x, y, z + 1)] | bc_mask[at(x - 1, y, z)] |
-        bc_mask[at(x, y - 1, z)] | bc_mask[at(x, y, z - 1)])
+    const Planes<uint8_t, O> m{reinterpret_cast<const char*>(bc_mask), 0};
+    if (*m.at(0, row + dx, zc) | *m.at(0, row, zc + 1) | *m.at(0, row + dy, zc) | *m.at(0, row - dx, zc) |
+        *m.at(0, row - dy, zc) | *m.at(0, row, zc - 1))
       return false;
   }
-  const T* ux = u;
-  const T* uy = u + N;
-  const T* uz = u + 2 * N;
-  const T h = T(2);
-  g.xx = (ux[at(x + 1, y, z)] - ux[at(x - 1, y, z)]) / h;
-  g.xy = (ux[at(x, y + 1, z)] - ux[at(x, y - 1, z)]) / h;
-  g.xz = (ux[at(x, y, z + 1)] - ux[at(x, y, z - 1)]) / h;
-  g.yx = (uy[at(x + 1, y, z)] - uy[at(x - 1, y, z)]) / h;
-  g.yy = (uy[at(x, y + 1, z)] - uy[at(x, y - 1, z)]) / h;
-  g.yz = (uy[at(x, y, z + 1)] - uy[at(x, y, z - 1)]) / h;
-  g.zx = (uz[at(x + 1, y, z)] - uz[at(x - 1, y, z)]) / h;
-  g.zy = (uz[at(x, y + 1, z)] - uz[at(x, y - 1, z)]) / h;
-  g.zz = (uz[at(x, y, z + 1)] - uz[at(x, y, z - 1)]) / h;
+  const Planes<T, O> v{reinterpret_cast<const char*>(u), (size_t)N * sizeof(T)};
+  auto d = [&](int comp, O plus_u, O plus_l, O minus_u, O minus_l) {
+    return (*v.at(comp, plus_u, plus_l) - *v.at(comp, minus_u, minus_l)) / T(2);
+  };
+  g.xx = d(0, row + dx, zc, row - dx, zc);
+  g.xy = d(0, row + dy, zc, row - dy, zc);
+  g.xz = d(0, row, zc + 1, row, zc - 1);
+  g.yx = d(1, row + dx, zc, row - dx, zc);
+  g.yy = d(1, row + dy, zc, row - dy, zc);
+  g.yz = d(1, row, zc + 1, row, zc - 1);
+  g.zx = d(2, row + dx, zc, row - dx, zc);
+  g.zy = d(2, row + dy, zc, row - dy, zc);
+  g.zz = d(2, row, zc + 1, row, zc - 1);
   return true;
 }
 
-template <class T>
+template <class T, class O>
 __global__ __launch_bounds__(256) void k_vorticity(const T* __restrict__ u, const uint8_t* __restrict__ bc_mask,
                                                    T* __restrict__ vort, T* __restrict__ mag, Sweep sw) {
   int x, y, z;
   if (!sweep_cell(sw, x, y, z)) return;
+  uniform_slow_axes(true, x, y);
   const uint64_t N = (uint64_t)sw.nx * sw.ny * sw.nz;
   Grad<T> g;
-  if (!central_gradient(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
-  const uint64_t cell = ((uint64_t)x * sw.ny + y) * sw.nz + z;
+  if (!central_gradient<T, O>(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
+  const O row = ((O)x * sw.ny + y) * sw.nz;
   const T vx = g.zy - g.yz;
   const T vy = g.xz - g.zx;
   const T vz = g.yx - g.xy;
-  vort[cell] = vx;
-  vort[N + cell] = vy;
-  vort[2 * N + cell] = vz;
-  mag[cell] = sqrt(vx * vx + vy * vy + vz * vz);
+  const Planes<T, O> vo{reinterpret_cast<const char*>(vort), (size_t)N * sizeof(T)};
+  const Planes<T, O> mo{reinterpret_cast<const char*>(mag), 0};
+  *vo.at_w(0, row, (O)z) = vx;
+  *vo.at_w(1, row, (O)z) = vy;
+  *vo.at_w(2, row, (O)z) = vz;
+  *mo.at_w(0, row, (O)z) = sqrt(vx * vx + vy * vy + vz * vz);
 }
 
-template <class T>
+template <class T, class O>
 __global__ __launch_bounds__(256) void k_q_criterion(const T* __restrict__ u, const uint8_t* __restrict__ bc_mask,
                                                      T* __restrict__ norm_mu, T* __restrict__ q, Sweep sw) {
   int x, y, z;
   if (!sweep_cell(sw, x, y, z)) return;
+  uniform_slow_axes(true, x, y);
   const uint64_t N = (uint64_t)sw.nx * sw.ny * sw.nz;
   Grad<T> g;
-  if (!central_gradient(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
-  const uint64_t cell = ((uint64_t)x * sw.ny + y) * sw.nz + z;
+  if (!central_gradient<T, O>(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
+  const O row = ((O)x * sw.ny + y) * sw.nz;
   const T mx = g.zy - g.yz, my = g.xz - g.zx, mz = g.yx - g.xy;
   const T mu = sqrt(mx * mx + my * my + mz * mz);
   const T half = T(0.5);
@@ -79,8 +88,9 @@ __global__ __launch_bounds__(256) void k_q_criterion(const T* __restrict__ u, co
   const T zero = T(0);
   const T oo = zero * zero + o01 * o01 + o02 * o02 + (-o01) * (-o01) + zero * zero + o12 * o12 + (-o02) * (-o02) +
                (-o12) * (-o12) + zero * zero;
-  norm_mu[cell] = mu;
-  q[cell] = half * (oo - ss);
+  const Planes<T, O> no{reinterpret_cast<const char*>(norm_mu), 0}, qo{reinterpret_cast<const char*>(q), 0};
+  *no.at_w(0, row, (O)z) = mu;
+  *qo.at_w(0, row, (O)z) = half * (oo - ss);
 }
 
 }  // namespace xlbh
@@ -98,13 +108,22 @@ int xlbh_vorticity(int field, const void* u, const uint8_t* bc_mask, void* vorti
   XLBH_REQUIRE(u && vorticity && magnitude, "vorticity needs u, vorticity and magnitude");
   const Sweep sw = make_sweep(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
-  if (field == XLBH_FIELD_F32)
-    hipLaunchKernelGGL(k_vorticity<float>, sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
-                       (float*)vorticity, (float*)magnitude, sw);
-  else if (field == XLBH_FIELD_F64)
-    hipLaunchKernelGGL(k_vorticity<double>, sweep_grid(sw), dim3(256), 0, s, (const double*)u, bc_mask,
-                       (double*)vorticity, (double*)magnitude, sw);
-  else {
+  const uint64_t N = (uint64_t)nx * ny * nz;
+  if (field == XLBH_FIELD_F32) {
+    if (3 * N * 4 <= 0xffffffffull)
+      hipLaunchKernelGGL((k_vorticity<float, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
+                         (float*)vorticity, (float*)magnitude, sw);
+    else
+      hipLaunchKernelGGL((k_vorticity<float, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
+                         (float*)vorticity, (float*)magnitude, sw);
+  } else if (field == XLBH_FIELD_F64) {
+    if (3 * N * 8 <= 0xffffffffull)
+      hipLaunchKernelGGL((k_vorticity<double, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
+                         bc_mask, (double*)vorticity, (double*)magnitude, sw);
+    else
+      hipLaunchKernelGGL((k_vorticity<double, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
+                         bc_mask, (double*)vorticity, (double*)magnitude, sw);
+  } else {
     set_error("vorticity supports float32 and float64 fields");
     return 1;
   }
@@ -121,13 +140,22 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
   XLBH_REQUIRE(u && norm_mu && q, "q_criterion needs u, norm_mu and q");
   const Sweep sw = make_sweep(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
-  if (field == XLBH_FIELD_F32)
-    hipLaunchKernelGGL(k_q_criterion<float>, sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
-                       (float*)norm_mu, (float*)q, sw);
-  else if (field == XLBH_FIELD_F64)
-    hipLaunchKernelGGL(k_q_criterion<double>, sweep_grid(sw), dim3(256), 0, s, (const double*)u, bc_mask,
-                       (double*)norm_mu, (double*)q, sw);
-  else {
+  const uint64_t N = (uint64_t)nx * ny * nz;
+  if (field == XLBH_FIELD_F32) {
+    if (3 * N * 4 <= 0xffffffffull)
+      hipLaunchKernelGGL((k_q_criterion<float, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
+                         (float*)norm_mu, (float*)q, sw);
+    else
+      hipLaunchKernelGGL((k_q_criterion<float, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
+                         (float*)norm_mu, (float*)q, sw);
+  } else if (field == XLBH_FIELD_F64) {
+    if (3 * N * 8 <= 0xffffffffull)
+      hipLaunchKernelGGL((k_q_criterion<double, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
+                         bc_mask, (double*)norm_mu, (double*)q, sw);
+    else
+      hipLaunchKernelGGL((k_q_criterion<double, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
+                         bc_mask, (double*)norm_mu, (double*)q, sw);
+  } else {
     set_error("q_criterion supports float32 and float64 fields");
     return 1;
   }
