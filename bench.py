@@ -752,6 +752,14 @@ def main():
     log(f"bench: device copy {copy['gbs']:.0f} GB/s")
     pull = pull_pattern_leg(xa, torch, xa.PrecisionPolicy.FP32FP32, n)
     log(f"bench: bare pull pattern (Stream) {pull['gbs']:.0f} GB/s ({pull['kernel_ms']:.4f} ms)")
+    if store16:
+        # the same pattern on 2-byte populations: the ceiling of the 16-bit step kernels
+        pull16 = pull_pattern_leg(xa, torch, xa.PrecisionPolicy.FP32FP16_SHIFTED, n)
+        log(f"bench: bare pull pattern, 2-byte populations {pull16['gbs']:.0f} GB/s ({pull16['kernel_ms']:.4f} ms)")
+        store16["pull_pattern_gbs"] = pull16["gbs"]
+        for name, rec in store16.items():
+            if isinstance(rec, dict) and "achieved_gbs" in rec:
+                rec["frac_of_pull_pattern"] = rec["achieved_gbs"] / pull16["gbs"]
     extras = {}
     if not args.no_extras:
         extras["periodic"] = periodic_leg(xa, torch, n, args.steps, args.warmup)
