@@ -322,6 +322,7 @@ CASES = {
     # name: (lattice class name, shape, collision, omega, init kind)
     "periodic": ("D3Q19", (24, 20, 16), "BGK", 1.6, "random"),
     "periodic_kbc": ("D3Q27", (16, 12, 10), "KBC", 1.9, "random"),
+    "periodic_q27": ("D3Q27", (16, 12, 10), "BGK", 1.7, "random"),
     "periodic_forced": ("D3Q19", (12, 10, 8), "BGK", 1.2, "random"),
     "periodic_field": ("D2Q9", (32, 24), "BGK", 1.3, "random"),
     "cavity_fullway": ("D3Q19", (20, 20, 20), "BGK", 1.0, "rest"),
@@ -368,7 +369,7 @@ LARGE_CASES = {
 BC_SET = {"mlups_cavity256": "cavity_fullway", "mlups_cavity64": "cavity_fullway", "sphere3d_bgk": "sphere3d", "c1_cavity64": "cavity_halfway", "c3_tgv": "periodic", "cavity_smagorinsky": "cavity_halfway",
           "channel2d_smagorinsky": "channel2d_zouhe", "sphere3d_smagorinsky": "sphere3d",
           "sphere3d_grads_bgk": "sphere3d_grads", "tiny_periodic3d": "periodic", "tiny_periodic2d": "periodic",
-          "tiny_periodic_kbc": "periodic", "tiny_cavity2d": "cavity2d", "tiny_cavity3d": "cavity_fullway",
+          "tiny_periodic_kbc": "periodic", "periodic_q27": "periodic", "tiny_cavity2d": "cavity2d", "tiny_cavity3d": "cavity_fullway",
           "long_periodic3d": "periodic", "long_periodic2d_kbc": "periodic", "long_cavity3d": "cavity_halfway",
           "long_cavity2d": "cavity2d"}
 
