@@ -701,7 +701,7 @@ def main():
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.size)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
+        print("bench.py measures one GPU only: the package has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
     import torch
