@@ -65,6 +65,11 @@ def test_error_channel_and_validation_without_gpu():
         rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
                          _lib.FORCE_NONE, None, None, nx, ny, nz, 0, 1.0, 0.0, 0, None)
         assert rc != 0 and msg in L.xlbh_last_error(), (nx, ny, nz, L.xlbh_last_error())
+    # the post-processing stencils sweep the same launch grid and take the same bounds
+    for fn in (L.xlbh_vorticity, L.xlbh_q_criterion):
+        for (nx, ny, nz), msg in [((65536, 3, 3), b"65535"), ((3, 65536, 129), b"tiles of y rows")]:
+            rc = fn(_lib.FIELD_F32, a, None, b, ctypes.c_void_p(0x3000), nx, ny, nz, None)
+            assert rc != 0 and msg in L.xlbh_last_error(), (nx, ny, nz, L.xlbh_last_error())
     # the largest 2-D grid that fits passes validation: 65535 x 65535 cells = 2^32 - 2^17 + 1
     rc = L.xlbh_run(_lib.D2Q9, _lib.BGK, _lib.F16, _lib.COMPUTE_F32, a, b, None, None, None, None,
                     _lib.FORCE_NONE, None, None, 65535, 65535, 1, 0, 1.0, 0.0, 0, 0, None)

@@ -103,8 +103,8 @@ int xlbh_vorticity(int field, const void* u, const uint8_t* bc_mask, void* vorti
                    int nz, void* stream) {
   clear_error();
   XLBH_REQUIRE(nx > 2 && ny > 2 && nz > 2, "vorticity needs a 3-D grid of at least 3 cells per axis");
-  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 32), "more than 2^32 - 1 cells per field");
-  XLBH_REQUIRE(nx <= 65535, "nx must be <= 65535");
+  // the cell-count and launch-grid bounds of the 3-D sweep
+  if (int e = check_grid(XLBH_D3Q19, nx, ny, nz)) return e;
   XLBH_REQUIRE(u && vorticity && magnitude, "vorticity needs u, vorticity and magnitude");
   const Sweep sw = make_sweep(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
@@ -135,8 +135,8 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
                      void* stream) {
   clear_error();
   XLBH_REQUIRE(nx > 2 && ny > 2 && nz > 2, "q_criterion needs a 3-D grid of at least 3 cells per axis");
-  XLBH_REQUIRE((int64_t)nx * ny * nz < ((int64_t)1 << 32), "more than 2^32 - 1 cells per field");
-  XLBH_REQUIRE(nx <= 65535, "nx must be <= 65535");
+  // the cell-count and launch-grid bounds of the 3-D sweep
+  if (int e = check_grid(XLBH_D3Q19, nx, ny, nz)) return e;
   XLBH_REQUIRE(u && norm_mu && q, "q_criterion needs u, norm_mu and q");
   const Sweep sw = make_sweep(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
