@@ -15,21 +15,28 @@ struct Grad {
   T xx, xy, xz, yx, yy, yz, zx, zy, zz;  // d u_a / d b as u_a_db
 };
 
-// Returns false when the cell is on the domain edge or next to a boundary cell.  x and y are
-// wave-uniform (uniform_slow_axes), so every neighbour's address is a scalar row base plus the
-// lane's z offset (Planes::at(l, uni, lane)); O = uint32_t while the three velocity planes stay
-// below 4 GiB.
-template <class T, class O>
+// Returns false when the cell is on the domain edge or, with MASK, next to a boundary cell.  x and
+// y are wave-uniform (uniform_slow_axes), so every neighbour's address is a scalar row base plus
+// the lane's z offset (Planes::at(l, uni, lane)); O = uint32_t while the three velocity planes stay
+// below 4 GiB.  At an interior cell every neighbour is in the grid, so the velocity loads are
+// issued together with the six mask loads instead of after their test: the mask values pass
+// through an empty asm placed after the velocity loads, which is where the test waits for them.
+// g is left unused where the test fails.
+template <class T, class O, bool MASK>
 __device__ __forceinline__ bool central_gradient(const T* u, const uint8_t* bc_mask, int x, int y, int z, int nx,
                                                  int ny, int nz, uint64_t N, Grad<T>& g) {
   if (x < 1 || y < 1 || z < 1 || x > nx - 2 || y > ny - 2 || z > nz - 2) return false;
   const O row = ((O)x * ny + y) * nz, dx = (O)ny * nz, dy = (O)nz;
   const O zc = (O)z;
-  if (bc_mask) {
-    const Planes<uint8_t, O> m{reinterpret_cast<const char*>(bc_mask), 0};
-    if (*m.at(0, row + dx, zc) | *m.at(0, row, zc + 1) | *m.at(0, row + dy, zc) | *m.at(0, row - dx, zc) |
-        *m.at(0, row - dy, zc) | *m.at(0, row, zc - 1))
-      return false;
+  uint32_t m[6] = {0, 0, 0, 0, 0, 0};
+  if constexpr (MASK) {
+    const Planes<uint8_t, O> mk{reinterpret_cast<const char*>(bc_mask), 0};
+    m[0] = *mk.at(0, row + dx, zc);
+    m[1] = *mk.at(0, row, zc + 1);
+    m[2] = *mk.at(0, row + dy, zc);
+    m[3] = *mk.at(0, row - dx, zc);
+    m[4] = *mk.at(0, row - dy, zc);
+    m[5] = *mk.at(0, row, zc - 1);
   }
   const Planes<T, O> v{reinterpret_cast<const char*>(u), (size_t)N * sizeof(T)};
   auto d = [&](int comp, O plus_u, O plus_l, O minus_u, O minus_l) {
@@ -44,10 +51,14 @@ __device__ __forceinline__ bool central_gradient(const T* u, const uint8_t* bc_m
   g.zx = d(2, row + dx, zc, row - dx, zc);
   g.zy = d(2, row + dy, zc, row - dy, zc);
   g.zz = d(2, row, zc + 1, row, zc - 1);
+  if constexpr (MASK) {
+    asm volatile("" : "+v"(m[0]), "+v"(m[1]), "+v"(m[2]), "+v"(m[3]), "+v"(m[4]), "+v"(m[5])::"memory");
+    return (m[0] | m[1] | m[2] | m[3] | m[4] | m[5]) == 0;
+  }
   return true;
 }
 
-template <class T, class O>
+template <class T, class O, bool MASK>
 __global__ __launch_bounds__(256) void k_vorticity(const T* __restrict__ u, const uint8_t* __restrict__ bc_mask,
                                                    T* __restrict__ vort, T* __restrict__ mag, Sweep sw) {
   int x, y, z;
@@ -55,7 +66,7 @@ __global__ __launch_bounds__(256) void k_vorticity(const T* __restrict__ u, cons
   uniform_slow_axes(true, x, y);
   const uint64_t N = (uint64_t)sw.nx * sw.ny * sw.nz;
   Grad<T> g;
-  if (!central_gradient<T, O>(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
+  if (!central_gradient<T, O, MASK>(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
   const O row = ((O)x * sw.ny + y) * sw.nz;
   const T vx = g.zy - g.yz;
   const T vy = g.xz - g.zx;
@@ -68,7 +79,7 @@ __global__ __launch_bounds__(256) void k_vorticity(const T* __restrict__ u, cons
   *mo.at_w(0, row, (O)z) = sqrt(vx * vx + vy * vy + vz * vz);
 }
 
-template <class T, class O>
+template <class T, class O, bool MASK>
 __global__ __launch_bounds__(256) void k_q_criterion(const T* __restrict__ u, const uint8_t* __restrict__ bc_mask,
                                                      T* __restrict__ norm_mu, T* __restrict__ q, Sweep sw) {
   int x, y, z;
@@ -76,7 +87,7 @@ __global__ __launch_bounds__(256) void k_q_criterion(const T* __restrict__ u, co
   uniform_slow_axes(true, x, y);
   const uint64_t N = (uint64_t)sw.nx * sw.ny * sw.nz;
   Grad<T> g;
-  if (!central_gradient<T, O>(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
+  if (!central_gradient<T, O, MASK>(u, bc_mask, x, y, z, sw.nx, sw.ny, sw.nz, N, g)) return;
   const O row = ((O)x * sw.ny + y) * sw.nz;
   const T mx = g.zy - g.yz, my = g.xz - g.zx, mz = g.yx - g.xy;
   const T mu = sqrt(mx * mx + my * my + mz * mz);
@@ -91,6 +102,25 @@ __global__ __launch_bounds__(256) void k_q_criterion(const T* __restrict__ u, co
   const Planes<T, O> no{reinterpret_cast<const char*>(norm_mu), 0}, qo{reinterpret_cast<const char*>(q), 0};
   *no.at_w(0, row, (O)z) = mu;
   *qo.at_w(0, row, (O)z) = half * (oo - ss);
+}
+
+// Calls fn(T{}, O{}, integral_constant<bool, MASK>{}) for the field's element type, the offset
+// type its three velocity planes need and whether a boundary mask is given; false for an unknown
+// field type.
+template <class Fn>
+bool with_stencil_types(int field, uint64_t N, bool mask, Fn&& fn) {
+  auto pick = [&](auto t) {
+    using T = decltype(t);
+    const bool narrow = 3 * N * sizeof(T) <= 0xffffffffull;
+    if (narrow && mask) fn(t, uint32_t{}, std::true_type{});
+    else if (narrow) fn(t, uint32_t{}, std::false_type{});
+    else if (mask) fn(t, uint64_t{}, std::true_type{});
+    else fn(t, uint64_t{}, std::false_type{});
+    return true;
+  };
+  if (field == XLBH_FIELD_F32) return pick(float{});
+  if (field == XLBH_FIELD_F64) return pick(double{});
+  return false;
 }
 
 }  // namespace xlbh
@@ -109,21 +139,12 @@ int xlbh_vorticity(int field, const void* u, const uint8_t* bc_mask, void* vorti
   const Sweep sw = make_sweep(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
   const uint64_t N = (uint64_t)nx * ny * nz;
-  if (field == XLBH_FIELD_F32) {
-    if (3 * N * 4 <= 0xffffffffull)
-      hipLaunchKernelGGL((k_vorticity<float, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
-                         (float*)vorticity, (float*)magnitude, sw);
-    else
-      hipLaunchKernelGGL((k_vorticity<float, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
-                         (float*)vorticity, (float*)magnitude, sw);
-  } else if (field == XLBH_FIELD_F64) {
-    if (3 * N * 8 <= 0xffffffffull)
-      hipLaunchKernelGGL((k_vorticity<double, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
-                         bc_mask, (double*)vorticity, (double*)magnitude, sw);
-    else
-      hipLaunchKernelGGL((k_vorticity<double, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
-                         bc_mask, (double*)vorticity, (double*)magnitude, sw);
-  } else {
+  const bool launched = with_stencil_types(field, N, bc_mask != nullptr, [&](auto t, auto o, auto m) {
+    using T = decltype(t);
+    hipLaunchKernelGGL((k_vorticity<T, decltype(o), decltype(m)::value>), sweep_grid(sw), dim3(256), 0, s,
+                       (const T*)u, bc_mask, (T*)vorticity, (T*)magnitude, sw);
+  });
+  if (!launched) {
     set_error("vorticity supports float32 and float64 fields");
     return 1;
   }
@@ -141,21 +162,12 @@ int xlbh_q_criterion(int field, const void* u, const uint8_t* bc_mask, void* nor
   const Sweep sw = make_sweep(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
   const uint64_t N = (uint64_t)nx * ny * nz;
-  if (field == XLBH_FIELD_F32) {
-    if (3 * N * 4 <= 0xffffffffull)
-      hipLaunchKernelGGL((k_q_criterion<float, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
-                         (float*)norm_mu, (float*)q, sw);
-    else
-      hipLaunchKernelGGL((k_q_criterion<float, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const float*)u, bc_mask,
-                         (float*)norm_mu, (float*)q, sw);
-  } else if (field == XLBH_FIELD_F64) {
-    if (3 * N * 8 <= 0xffffffffull)
-      hipLaunchKernelGGL((k_q_criterion<double, uint32_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
-                         bc_mask, (double*)norm_mu, (double*)q, sw);
-    else
-      hipLaunchKernelGGL((k_q_criterion<double, uint64_t>), sweep_grid(sw), dim3(256), 0, s, (const double*)u,
-                         bc_mask, (double*)norm_mu, (double*)q, sw);
-  } else {
+  const bool launched = with_stencil_types(field, N, bc_mask != nullptr, [&](auto t, auto o, auto m) {
+    using T = decltype(t);
+    hipLaunchKernelGGL((k_q_criterion<T, decltype(o), decltype(m)::value>), sweep_grid(sw), dim3(256), 0, s,
+                       (const T*)u, bc_mask, (T*)norm_mu, (T*)q, sw);
+  });
+  if (!launched) {
     set_error("q_criterion supports float32 and float64 fields");
     return 1;
   }
