@@ -26,8 +26,8 @@ Also reported on the same line:
 roofline.achieved = algorithmic bytes per launch (2 Q S_store per cell) / average kernel duration
 from HIP events on the launch stream; roofline.traffic = FETCH_SIZE + WRITE_SIZE per launch from
 rocprofv3 (profiles/traffic_step_d3q19_cavity256.json, written by --summarize-pmc).
-cpu_baseline = the C++/OpenMP oracle (the parity checker, a restatement of the reference
-algorithm) timed on a bounded number of steps of the headline workload on the host cores.
+cpu_baseline = the C++/OpenMP oracle (the parity reference of the tests, a restatement of the
+reference algorithm) timed on a bounded number of steps of the headline workload on the host cores.
 
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--quick]

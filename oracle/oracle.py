@@ -1,6 +1,6 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the parity checker of the HIP backend and the CPU baseline timed by
+TEST INFRASTRUCTURE ONLY: the parity reference of the HIP backend and the CPU baseline timed by
 bench.py.  Nothing in the product package imports this module.
 
 The oracle takes plain NumPy arrays in the reference's layout: populations (q, nx, ny, nz) in the

@@ -1,7 +1,7 @@
 // xlb_oracle.cpp -- CPU restatement of XLB's Warp-backend algorithm for the fused LBM step.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the parity checker (and the CPU baseline timed beside
-// the GPU): only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+// TEST INFRASTRUCTURE ONLY.  This file is the parity reference of the tests (and the CPU baseline
+// timed beside the GPU): only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
 // product (xlb_amd, libxlbhip.so) never calls it.
 //
 // It follows the reference's per-cell functionals line by line, including the reference's own

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the parity checker.
+"""Pins the CPU oracle before it is trusted as the parity reference.
 
 Checks come from (a) the reference's own known-answer tests (tests/kernels/*, tests/boundary_conditions/*),
 (b) lattice identities, (c) conservation laws and closed-form flows (plane Poiseuille, Taylor-Green
