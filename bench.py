@@ -457,6 +457,76 @@ def periodic_leg(xa, torch, n, steps, warmup):
     return res
 
 
+def slab_leg(xa, torch, n=256, reps=50):
+    """One GPU's share of an x-sharded run: an x-slab of n planes of n x n cells plus a ghost
+    plane on each side, stepped with xlbh_step_slab (D3Q19 BGK FP32, periodic box data), beside
+    the whole-domain n^3 step.  The slab's populations are timed with planes of the slab's own
+    size (dense) and with the library's recommended plane stride (xlbh_plane_stride, which pads a
+    plane just above a power of two); then, at the recommended stride, the two edge planes and
+    the interior as separate calls, and the ghost fill: the c_x = +1 / -1 populations of one
+    plane each (the data a sharded run exchanges per step and face)."""
+    from xlb_amd.workloads import periodic_box_3d
+
+    wl = periodic_box_3d(n, xa.PrecisionPolicy.FP32FP32, omega=1.7)
+    st = wl.stepper
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    res = dict(workload=f"D3Q19 BGK FP32, x-slab of {n} planes + 2 ghost planes ({n + 2}x{n}x{n}) vs the {n}^3 "
+                        "whole-domain step (periodic box data)")
+    res["whole_domain_us"] = timed(lambda: st(wl.f_0, wl.f_1, wl.bc_mask, wl.missing_mask, wl.omega, 0))
+    idx = torch.arange(-1, n + 1, device="cuda") % n
+    f_src = wl.f_0[:, idx].contiguous()
+    bm, mm = wl.bc_mask[:, idx].contiguous(), wl.missing_mask[:, idx].contiguous()
+    del wl
+    torch.cuda.empty_cache()
+    plane_cells = n * n
+    from xlb_amd import _lib
+
+    recommended = int(_lib.lib().xlbh_plane_stride(_lib.D3Q19, _lib.F32, n + 2, n, n))
+    for label, stride in (("dense", (n + 2) * plane_cells), ("recommended", recommended)):
+        bufs = [torch.empty(19 * stride, dtype=torch.float32, device="cuda") for _ in range(2)]
+        a, b = [u.as_strided(tuple(f_src.shape), (stride, plane_cells, n, 1)) for u in bufs]
+        a.copy_(f_src)
+        b.copy_(f_src)
+        us = timed(lambda: st.step_slab(a, b, bm, mm, 1.7, 0, 1, n + 1))
+        res[f"slab_{label}_us"] = us
+        res[f"slab_{label}_plane_mib"] = stride * 4 / 2**20
+        res[f"slab_{label}_mlups"] = n**3 / us
+        if label == "recommended":
+            res["edge_planes_us"] = timed(lambda: (st.step_slab(a, b, bm, mm, 1.7, 0, 1, 2),
+                                                   st.step_slab(a, b, bm, mm, 1.7, 0, n, n + 1)))
+            res["interior_us"] = timed(lambda: st.step_slab(a, b, bm, mm, 1.7, 0, 2, n))
+            cx = np.asarray(st.velocity_set.c)[0]
+            up = [int(l) for l in np.nonzero(cx == 1)[0]]
+            dn = [int(l) for l in np.nonzero(cx == -1)[0]]
+
+            def fill():
+                for l in up:
+                    b[l, 0].copy_(a[l, n])
+                for l in dn:
+                    b[l, n + 1].copy_(a[l, 1])
+
+            res["ghost_fill_us"] = timed(fill)
+            res["ghost_fill_bytes"] = (len(up) + len(dn)) * plane_cells * 4
+        del a, b, bufs
+        torch.cuda.empty_cache()
+    log(f"bench: slab share {res['slab_recommended_us']:.1f} us (planes {res['slab_recommended_plane_mib']:.1f} MiB), "
+        f"{res['slab_dense_us']:.1f} us (dense), whole domain {res['whole_domain_us']:.1f} us")
+    return res
+
+
 def copy_leg(torch, nbytes=2 * 19 * 4 * 256**3 // 2, reps=20):
     """Device-to-device copy rate (torch copy_ of one 1.27 GB buffer into another): the practical
     streaming ceiling the step kernel is compared with."""
@@ -695,7 +765,7 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--only", default=None,
                     help="comma-separated extra legs to run alone (kbc_sphere, grads_sphere, les_cavity, mesh_masker, "
-                         "adjoint, periodic), printing their JSON; for profiling")
+                         "adjoint, periodic, precision_viscosity_sweep, slab), printing their JSON; for profiling")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.size)
@@ -720,6 +790,7 @@ def main():
             "adjoint": lambda: adjoint_leg(xa, torch, quick=args.quick),
             "periodic": lambda: periodic_leg(xa, torch, n, args.steps, args.warmup),
             "precision_viscosity_sweep": lambda: precision_viscosity_sweep(xa, torch),
+            "slab": lambda: slab_leg(xa, torch, n),
         }
         names = args.only.split(",")
         unknown = [k for k in names if k not in legs]

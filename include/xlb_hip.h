@@ -117,8 +117,11 @@ const char* xlbh_last_error(void);
 int xlbh_abi_version(void);
 
 /* Plane stride (elements) this library recommends for population fields of the given grid and
- * store format: N plus a pad that staggers the Q plane starts across HBM channels (-1 for
- * invalid arguments).  No counterpart in the reference (WarpGrid fields are dense). */
+ * store format (-1 for invalid arguments): N plus a pad that staggers the Q plane starts across
+ * HBM channels (256 bytes for 2-byte populations), except that a plane of more than 16 MiB
+ * which exceeds a power of two 2^k by less than 1 MiB (an x-slab of a power-of-two share with
+ * its ghost planes) is padded to 2^k + 1 MiB.  No counterpart in the reference (WarpGrid fields
+ * are dense). */
 int64_t xlbh_plane_stride(int lattice, int store, int nx, int ny, int nz);
 
 /* Bytes of one population field with plane stride `plane` (0 = N): Q * plane * S, and for
@@ -154,6 +157,25 @@ int xlbh_step(int lattice, int collision, int store, int compute,
               const void* aux, const xlbh_bc_desc* bc_table,
               int force_mode, const double* force_uniform, const void* force_field,
               int nx, int ny, int nz, int64_t plane, double omega, double collision_param, int timestep, void* stream);
+
+/* One fused step of an x-slab: xlbh_step restricted to planes x_begin <= x < x_end, with
+ * 1 <= x_begin < x_end <= nx - 1, so that no pull wraps in x.  Planes x_begin - 1 and x_end are
+ * read as the x-neighbours (ghost planes, which the caller fills with the adjacent slab's
+ * populations before the call) and every plane outside the range is left unwritten.  All other
+ * arguments, and the per-cell arrays (bc_mask, missing_bits, aux, force_field), are those of the
+ * slab's own (nx, ny, nz) field, ghost planes included.  The per-shard operator of the
+ * reference's x-sharded run (distribute.py:18-47, where each shard steps its block and the
+ * populations that crossed a shard face are exchanged afterwards with lax.ppermute); the pull
+ * scheme needs the crossing populations before the step, hence the ghost planes.  Calling it
+ * for the two edge planes and then for the interior lets the exchange of the edge planes overlap
+ * the interior's update.  The block format is refused. */
+int xlbh_step_slab(int lattice, int collision, int store, int compute,
+                   const void* f_in, void* f_out,
+                   const uint8_t* bc_mask, const uint32_t* missing_bits,
+                   const void* aux, const xlbh_bc_desc* bc_table,
+                   int force_mode, const double* force_uniform, const void* force_field,
+                   int nx, int ny, int nz, int64_t plane, double omega, double collision_param, int timestep,
+                   int x_begin, int x_end, void* stream);
 
 /* n_steps fused steps with the buffer swap done in C (the caller's loop of
  * mlups_3d.py:78-80 moved below the ABI).  Step t reads f_a when t is even and f_b when t is

@@ -65,6 +65,14 @@ def test_error_channel_and_validation_without_gpu():
         rc = L.xlbh_step(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
                          _lib.FORCE_NONE, None, None, nx, ny, nz, 0, 1.0, 0.0, 0, None)
         assert rc != 0 and msg in L.xlbh_last_error(), (nx, ny, nz, L.xlbh_last_error())
+    # the slab step: its planes need a ghost plane on each side; the block format is refused
+    for xb, xe in ((0, 4), (3, 3), (5, 2), (1, 8)):
+        rc = L.xlbh_step_slab(_lib.D3Q19, _lib.BGK, _lib.F32, _lib.COMPUTE_F32, a, b, None, None, None, None,
+                              _lib.FORCE_NONE, None, None, 8, 4, 4, 0, 1.0, 0.0, 0, xb, xe, None)
+        assert rc != 0 and b"ghost plane" in L.xlbh_last_error(), (xb, xe, L.xlbh_last_error())
+    rc = L.xlbh_step_slab(_lib.D3Q19, _lib.BGK, _lib.I16_BLOCK, _lib.COMPUTE_F32, a, b, None, None, None, None,
+                          _lib.FORCE_NONE, None, None, 8, 4, 4, 0, 1.0, 0.0, 0, 1, 7, None)
+    assert rc != 0 and b"block format" in L.xlbh_last_error()
     # the post-processing stencils sweep the same launch grid and take the same bounds
     for fn in (L.xlbh_vorticity, L.xlbh_q_criterion):
         for (nx, ny, nz), msg in [((65536, 3, 3), b"65535"), ((3, 65536, 129), b"tiles of y rows")]:
@@ -140,3 +148,24 @@ def test_population_layout_sizes():
         off = -(-(q * n * 2) // 256) * 256
         assert L.xlbh_population_bytes(lat, _lib.I16_BLOCK, nx, ny, nz, 0) == off + 4 * segs
     assert L.xlbh_population_bytes(_lib.D3Q19, _lib.F32, 4, 4, 4, 63) == -1
+
+
+def test_recommended_plane_stride():
+    """xlbh_plane_stride: N for 4- and 8-byte populations and N + 128 for 2-byte ones, except for
+    planes of more than 16 MiB that exceed a power of two by less than 1 MiB (an x-slab of a
+    power-of-two share with its two ghost planes), which are padded to 2^k + 1 MiB."""
+    from xlb_amd import _lib
+
+    L = _lib.lib()
+    MiB = 1 << 20
+    n3 = 256**3
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F32, 256, 256, 256) == n3
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F16_SHIFTED, 256, 256, 256) == n3 + 128
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F64, 240, 240, 240) == 240**3
+    # the slab of a 256^3 share: 64 MiB + 512 KiB of FP32, 32 MiB + 256 KiB of FP16
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F32, 258, 256, 256) == (64 * MiB + MiB) // 4
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F16_SHIFTED, 258, 256, 256) == (32 * MiB + MiB) // 2
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F64, 130, 128, 128) == (16 * MiB + MiB) // 8
+    # exceeding the power of two by 1 MiB or more, or small planes: unchanged
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F32, 260, 256, 256) == 260 * 256 * 256
+    assert L.xlbh_plane_stride(_lib.D3Q19, _lib.F32, 130, 128, 128) == 130 * 128 * 128

@@ -62,6 +62,8 @@ SIGNATURES = {
     "xlbh_population_bytes": (_i64, [_i, _i, _i, _i, _i, _i64]),
     "xlbh_step": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _i64, _d, _d, _i, _vp]),
     "xlbh_run": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _i64, _d, _d, _i, _i, _vp]),
+    "xlbh_step_slab": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _vp, _i, _i, _i, _i64, _d, _d, _i, _i,
+                            _i, _vp]),
     "xlbh_stream": (_i, [_i, _i, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_equilibrium": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),
     "xlbh_moments": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i64, _vp]),

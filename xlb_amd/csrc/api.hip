@@ -42,9 +42,21 @@ int resolve_plane(int lattice, int store, int nx, int ny, int nz, int64_t plane,
 // +2 to +4.7 %.  With 4-byte populations no pad from 256 B to 1 MB helped (0 to -4 %).
 static uint64_t plane_pad_bytes(int store) { return store_bytes(store) == 2 ? 256 : 0; }
 
+// Planes a little longer than a power of two stream badly: an x-slab of a 256^3 share with its
+// two ghost planes has FP32 planes of 64 MiB + 512 KiB, and its D3Q19 step took 516-916 us
+// against 430 us for 256^3 (shifted FP16: 32 MiB + 256 KiB, 262-273 us against 225 us; FP64 at
+// 128^2: 16 MiB + 256 KiB, 120 us against 113 us).  Padded to 2^k + 1 MiB they step at the
+// whole-domain rate (435, 223 and 113 us).  Sweeping the pad after a 64 MiB plane put the slow
+// band at about 320-640 KiB.  So a plane of more than 16 MiB that exceeds a power of two 2^k by
+// less than 1 MiB is padded to 2^k + 1 MiB; other sizes keep the 2-byte pad above.
 static int64_t recommended_plane(int store, int nx, int ny, int nz) {
   const uint64_t N = (uint64_t)nx * ny * nz;
-  return (int64_t)(N + plane_pad_bytes(store) / store_bytes(store));
+  const uint64_t S = store_bytes(store), dense = N * S, MiB = 1ull << 20;
+  if (dense > 16 * MiB) {
+    const uint64_t p2 = 1ull << (63 - __builtin_clzll(dense));
+    if (dense > p2 && dense - p2 < MiB) return (int64_t)((p2 + MiB) / S);
+  }
+  return (int64_t)(N + plane_pad_bytes(store) / S);
 }
 
 static int dispatch_step(int lattice, const StepCall& c) {
@@ -120,6 +132,29 @@ int xlbh_step(int lattice, int collision, int store, int compute, const void* f_
   if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
   StepCall c{collision, store, compute, f_in, f_out, bc_mask, missing_bits, aux, bc_table,
              force_mode, force_uniform, force_field, nx, ny, nz, P, omega, collision_param, (hipStream_t)stream};
+  return dispatch_step(lattice, c);
+}
+
+int xlbh_step_slab(int lattice, int collision, int store, int compute, const void* f_in, void* f_out,
+                   const uint8_t* bc_mask, const uint32_t* missing_bits, const void* aux,
+                   const xlbh_bc_desc* bc_table, int force_mode, const double* force_uniform,
+                   const void* force_field, int nx, int ny, int nz, int64_t plane, double omega,
+                   double collision_param, int timestep, int x_begin, int x_end, void* stream) {
+  (void)timestep;
+  clear_error();
+  if (int e = validate_step(lattice, collision, store, compute, f_in, f_out, bc_mask, missing_bits, bc_table,
+                            force_mode, force_uniform, force_field, nx, ny, nz))
+    return e;
+  XLBH_REQUIRE(x_begin >= 1 && x_begin < x_end && x_end <= nx - 1,
+               "step_slab: planes x_begin..x_end-1 need 1 <= x_begin < x_end <= nx - 1 (a ghost plane on each side)");
+  XLBH_REQUIRE(store != XLBH_I16_BLOCK,
+               "step_slab: the block format is not supported (its row exponents would need an exchange of their own)");
+  uint64_t P;
+  if (int e = resolve_plane(lattice, store, nx, ny, nz, plane, P)) return e;
+  StepCall c{collision, store, compute, f_in, f_out, bc_mask, missing_bits, aux, bc_table,
+             force_mode, force_uniform, force_field, nx, ny, nz, P, omega, collision_param, (hipStream_t)stream};
+  c.x0 = x_begin;
+  c.x1 = x_end;
   return dispatch_step(lattice, c);
 }
 

@@ -260,21 +260,23 @@ struct FieldT<XLBH_FIELD_BF16> {
 // 3-D: fast = z, mid = y, slow = x.  2-D (nz == 1): fast = y, mid = x.
 // ------------------------------------------------------------------------------------------
 struct Sweep {
-  int nx, ny, nz;
-  int lbz;  // log2(bz)
+  int nx, ny, nz;  // the field's extents
+  int lbz;         // log2(bz)
+  int x0, x1;      // the planes x0 <= x < x1 the sweep visits (all of them unless restricted)
 };
 
 inline Sweep make_sweep(int nx, int ny, int nz) {
   const int fast = (nz > 1) ? nz : ny;
   int lbz = 6;
   while (lbz < 8 && (1 << lbz) < fast) ++lbz;
-  return Sweep{nx, ny, nz, lbz};
+  return Sweep{nx, ny, nz, lbz, 0, nx};
 }
 
 inline dim3 sweep_grid(const Sweep& s) {
   const int bz = 1 << s.lbz, by = 256 / bz;
-  if (s.nz > 1) return dim3((s.nz + bz - 1) / bz, (s.ny + by - 1) / by, s.nx);
-  return dim3((s.ny + bz - 1) / bz, (s.nx + by - 1) / by, 1);
+  const int xs = s.x1 - s.x0;
+  if (s.nz > 1) return dim3((s.nz + bz - 1) / bz, (s.ny + by - 1) / by, xs);
+  return dim3((s.ny + bz - 1) / bz, (xs + by - 1) / by, 1);
 }
 
 // Thread -> cell coordinates of tile (bx, by, bz); false when the thread lies outside the grid.
@@ -286,13 +288,13 @@ __device__ __forceinline__ bool sweep_cell_of(const Sweep& s, int bx, int by, in
   if (s.nz > 1) {
     z = fast;
     y = mid;
-    x = bzi;
+    x = s.x0 + bzi;
     return z < s.nz && y < s.ny;
   }
   z = 0;
   y = fast;
-  x = mid;
-  return y < s.ny && x < s.nx;
+  x = s.x0 + mid;
+  return y < s.ny && x < s.x1;
 }
 __device__ __forceinline__ bool sweep_cell(const Sweep& s, int& x, int& y, int& z) {
   return sweep_cell_of(s, blockIdx.x, blockIdx.y, blockIdx.z, x, y, z);

@@ -365,6 +365,9 @@ struct StepCall {
   double omega;
   double cparam;  // Smagorinsky constant (XLBH_SMAGORINSKY_BGK)
   hipStream_t stream;
+  // Planes x0 <= x < x1 to update (x1 < 0: all).  With 1 <= x0 and x1 <= nx - 1 no pull wraps
+  // in x: planes x0 - 1 and x1 are read as neighbours and left unwritten (ghost planes).
+  int x0 = 0, x1 = -1;
 };
 
 template <int LAT, int COLL, int FMT, class C>
@@ -382,6 +385,10 @@ int launch_step_t(const StepCall& c) {
   p.omega = c.omega;
   p.cparam = c.cparam;
   p.sw = make_sweep(c.nx, c.ny, c.nz);
+  if (c.x1 >= 0) {
+    p.sw.x0 = c.x0;
+    p.sw.x1 = c.x1;
+  }
   p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
   p.plane = c.plane;
   p.e_in = nullptr;

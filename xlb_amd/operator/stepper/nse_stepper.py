@@ -231,6 +231,27 @@ class IncompressibleNavierStokesStepper(Stepper):
         _lib.mark_written(f_1)
         return f_0, f_1
 
+    def step_slab(self, f_0, f_1, bc_mask, missing_mask, omega, timestep, x_begin, x_end):
+        """One fused step of an x-slab (``xlbh_step_slab``): only planes x_begin <= x < x_end of
+        f_1 are written, with 1 <= x_begin < x_end <= nx - 1.  Planes x_begin - 1 and x_end of
+        f_0 are read as ghost planes; they must hold the adjacent slabs' populations (those with
+        c_x = +1 in the lower ghost plane and c_x = -1 in the upper one are the ones read).  The
+        fields, masks and boundary conditions are the slab's own, ghost planes included.  This is
+        the per-shard step of an x-sharded run (the reference's distribute.py:18-47 exchanges the
+        crossing populations after each shard's step; the pull scheme needs them before it)."""
+        nx, ny, nz, plane, masks, force_mode, fu_ptr, keep, force_field = self._launch_args(f_0, f_1, bc_mask, missing_mask)
+        _lib.check(
+            _lib.lib().xlbh_step_slab(
+                self.lattice_id, self._collision_id(), self.precision_policy.store_format, self.compute_id,
+                _lib.ptr(f_0), _lib.ptr(f_1), *masks, force_mode, fu_ptr, _lib.ptr(force_field),
+                nx, ny, nz, plane, float(omega), self._collision_param(), int(timestep), int(x_begin), int(x_end),
+                _lib.stream_of(f_0),
+            ),
+            "IncompressibleNavierStokesStepper.step_slab",
+        )
+        _lib.mark_written(f_1)
+        return f_0, f_1
+
     _ADJOINT_BC_KINDS = (
         _lib.BC_HALFWAY_BB, _lib.BC_EXTRAPOLATION_OUTFLOW, _lib.BC_DO_NOTHING, _lib.BC_ZOUHE_VELOCITY,
         _lib.BC_ZOUHE_PRESSURE, _lib.BC_REGULARIZED_VELOCITY, _lib.BC_REGULARIZED_PRESSURE,
