@@ -260,24 +260,25 @@ struct FieldT<XLBH_FIELD_BF16> {
 // 3-D: fast = z, mid = y, slow = x.  2-D (nz == 1): fast = y, mid = x.
 // ------------------------------------------------------------------------------------------
 struct Sweep {
-  int nx, ny, nz;  // the field's extents
-  int lbz;         // log2(bz)
-  int x0, x1;      // the planes x0 <= x < x1 the sweep visits (all of them unless restricted)
+  int nx, ny, nz;
+  int lbz;  // log2(bz)
 };
 
 inline Sweep make_sweep(int nx, int ny, int nz) {
   const int fast = (nz > 1) ? nz : ny;
   int lbz = 6;
   while (lbz < 8 && (1 << lbz) < fast) ++lbz;
-  return Sweep{nx, ny, nz, lbz, 0, nx};
+  return Sweep{nx, ny, nz, lbz};
 }
 
-inline dim3 sweep_grid(const Sweep& s) {
+// The sweep's launch grid over planes x0 <= x < x1 of the grid (sweep_grid: all planes).
+inline dim3 sweep_grid_range(const Sweep& s, int x0, int x1) {
   const int bz = 1 << s.lbz, by = 256 / bz;
-  const int xs = s.x1 - s.x0;
-  if (s.nz > 1) return dim3((s.nz + bz - 1) / bz, (s.ny + by - 1) / by, xs);
-  return dim3((s.ny + bz - 1) / bz, (xs + by - 1) / by, 1);
+  if (s.nz > 1) return dim3((s.nz + bz - 1) / bz, (s.ny + by - 1) / by, x1 - x0);
+  return dim3((s.ny + bz - 1) / bz, (x1 - x0 + by - 1) / by, 1);
 }
+
+inline dim3 sweep_grid(const Sweep& s) { return sweep_grid_range(s, 0, s.nx); }
 
 // Thread -> cell coordinates of tile (bx, by, bz); false when the thread lies outside the grid.
 __device__ __forceinline__ bool sweep_cell_of(const Sweep& s, int bx, int by, int bzi, int& x, int& y, int& z) {
@@ -288,16 +289,22 @@ __device__ __forceinline__ bool sweep_cell_of(const Sweep& s, int bx, int by, in
   if (s.nz > 1) {
     z = fast;
     y = mid;
-    x = s.x0 + bzi;
+    x = bzi;
     return z < s.nz && y < s.ny;
   }
   z = 0;
   y = fast;
-  x = s.x0 + mid;
-  return y < s.ny && x < s.x1;
+  x = mid;
+  return y < s.ny && x < s.nx;
 }
 __device__ __forceinline__ bool sweep_cell(const Sweep& s, int& x, int& y, int& z) {
   return sweep_cell_of(s, blockIdx.x, blockIdx.y, blockIdx.z, x, y, z);
+}
+// sweep_cell for a launch of sweep_grid_range(s, x0, x1).
+__device__ __forceinline__ bool sweep_cell_range(const Sweep& s, int x0, int x1, int& x, int& y, int& z) {
+  const bool in = sweep_cell_of(s, blockIdx.x, blockIdx.y, blockIdx.z, x, y, z);
+  x += x0;
+  return in && x < x1;
 }
 
 __device__ __forceinline__ int wrap(int v, int n) {

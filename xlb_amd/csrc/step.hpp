@@ -38,6 +38,7 @@ struct StepParams {
   uint64_t plane;        // population plane stride (elements)
   const int32_t* e_in;   // XLBH_I16_BLOCK: segment exponents of f_in and f_out
   int32_t* e_out;
+  int x0, x1;            // the planes x0 <= x < x1 this launch updates
 };
 
 // Reads of stored populations other than the pulls: the own cell's pre-stream values and the
@@ -243,7 +244,7 @@ __device__ __forceinline__ void step_body(const StepParams& p) {
   using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
 
   int x, y, z;
-  const bool live = sweep_cell(p.sw, x, y, z);
+  const bool live = sweep_cell_range(p.sw, p.x0, p.x1, x, y, z);
   const int nx = p.sw.nx, ny = p.sw.ny, nz = p.sw.nz;
   if constexpr (!BLK) {
     if (!live) return;
@@ -385,10 +386,8 @@ int launch_step_t(const StepCall& c) {
   p.omega = c.omega;
   p.cparam = c.cparam;
   p.sw = make_sweep(c.nx, c.ny, c.nz);
-  if (c.x1 >= 0) {
-    p.sw.x0 = c.x0;
-    p.sw.x1 = c.x1;
-  }
+  p.x0 = c.x1 >= 0 ? c.x0 : 0;
+  p.x1 = c.x1 >= 0 ? c.x1 : c.nx;
   p.N = (uint32_t)((int64_t)c.nx * c.ny * c.nz);
   p.plane = c.plane;
   p.e_in = nullptr;
@@ -405,11 +404,11 @@ int launch_step_t(const StepCall& c) {
   auto launch = [&](auto mseg) {
     constexpr bool MS = decltype(mseg)::value;
     if (wide)
-      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true, MS>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, true, MS>), sweep_grid_range(p.sw, p.x0, p.x1), dim3(256), lds, c.stream, p);
     else if constexpr (w6)
-      hipLaunchKernelGGL((k_step_w6<LAT, COLL, FMT, C, false, MS>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+      hipLaunchKernelGGL((k_step_w6<LAT, COLL, FMT, C, false, MS>), sweep_grid_range(p.sw, p.x0, p.x1), dim3(256), lds, c.stream, p);
     else
-      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false, MS>), sweep_grid(p.sw), dim3(256), lds, c.stream, p);
+      hipLaunchKernelGGL((k_step<LAT, COLL, FMT, C, false, MS>), sweep_grid_range(p.sw, p.x0, p.x1), dim3(256), lds, c.stream, p);
   };
   if constexpr (FMT == XLBH_I16_BLOCK) {
     if (blk_segs_fast(p.sw) > 1)
