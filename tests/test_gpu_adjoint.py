@@ -378,3 +378,16 @@ def test_rollout_keeps_its_own_force_field():
         fd = central(lambda t: c.loss_at(c.f0, steps, ff + t * v), 1e-6, False)
         ad = float((ref_g * v).sum())
         assert rel(ad, fd) <= FD_DIRECTION_TOL, f"adjoint {ad:.10e} vs FD {fd:.10e}"
+
+
+def test_fp64_q27_reverse_is_bit_stable():
+    """The FP64 D3Q27 reverse kernels use every VGPR and spill to scratch.  Their fused path carries
+    the boundary contributions in a workspace (no atomics), so its gradients are bit-identical
+    between two identical rollouts and with or without the next segment recorded on a second
+    stream; a build whose code for these kernels went wrong showed run-to-run differences here."""
+    c = Case("smagorinsky_q27", P.FP64FP64, seed=6)
+    a, _ = c.gradients(9, every=4, fused=True, overlap=True)
+    b, _ = c.gradients(9, every=4, fused=True, overlap=True)
+    n, _ = c.gradients(9, every=4, fused=True, overlap=False)
+    assert torch.equal(a, b)
+    assert torch.equal(a, n)
