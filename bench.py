@@ -23,6 +23,9 @@ Also reported on the same line:
 * les_cavity -- the LES lid-driven cavity of examples/out_of_core/lid_driven_cavity.py (D3Q19
   SmagorinskyLESBGK, Cs = 0.17, 256^3, lid u = 0.06, tau = 0.501, FP32), with the Q-criterion
   of its velocity field (examples' RenderQCriterionSubroutine) timed on the final state.
+* slab -- one GPU's share of an x-sharded run: the 256^3 periodic D3Q19 BGK FP32 data as a slab
+  of 256 planes plus two ghost planes, stepped with xlbh_step_slab at dense and at the library's
+  recommended plane stride, beside the whole-domain step; edge planes, interior and ghost fill.
 roofline.achieved = algorithmic bytes per launch (2 Q S_store per cell) / average kernel duration
 from HIP events on the launch stream; roofline.traffic = FETCH_SIZE + WRITE_SIZE per launch from
 rocprofv3 (profiles/traffic_step_d3q19_cavity256.json, written by --summarize-pmc).
@@ -846,6 +849,7 @@ def main():
         extras["grads_sphere"] = kbc_sphere_leg(xa, torch, quick=args.quick, sphere_bc="grads")
         extras["mesh_masker"] = mesh_leg(xa, torch, subdivisions=4 if args.quick else 6)
         extras["les_cavity"] = les_cavity_leg(xa, torch, 128 if args.quick else n, args.steps, args.warmup)
+        extras["slab"] = slab_leg(xa, torch, 128 if args.quick else n)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline (oracle) ...")
